@@ -1,0 +1,278 @@
+// dfft_axis.hpp -- device body of one axis-pass tile (HIP, gfx950).
+//
+// A tile is TC lines of one axis, transformed by a Stockham autosort FFT
+// (Stockham 1966; the formulation of Govindaraju et al., SC'08): stage s with
+// radix R and Ns = R_0*...*R_{s-1} reads x[j + r*L/R], twiddles by
+// w_{Ns*R}^{r*(j mod Ns)}, runs an R-point DFT and writes
+// y[(j - j mod Ns)*R + j mod Ns + r*Ns].  Stage 0 reads straight from HBM and
+// the last stage writes straight to HBM in natural order; both patterns are
+// unit-stride across threads, so every global access is coalesced.  The
+// stages in between exchange through LDS.
+//
+// Two layouts:
+//   CONTIG   lines are contiguous (axis 0): a line's TPL threads are adjacent
+//            lanes and walk the line; LDS holds one padded slot per line.
+//   STRIDED  lines run across rows (axes 1, 2): lanes are adjacent columns,
+//            so each row segment of TC elements is one contiguous burst; LDS
+//            is row-major [p][column].
+//
+// The fused ops (R2C/C2R half-length packing, DCT-II/III reordering and
+// twiddles) live in the stage-0 loader and the last-stage storer.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "dfft_math.hpp"
+#include "dfft_types.hpp"
+
+namespace dfft {
+
+template <class T, int OP, bool CONTIG>
+struct TileCtx {
+    const AxisPass &a;
+    cx<T> *lds;
+    int ls, t;
+    bool valid;
+    int64_t ib, ob;
+
+    __device__ __forceinline__ int li(int p) const {
+        if constexpr (CONTIG) return ls * a.lds_pitch + p + (p >> a.pad_shift);
+        else return p * a.lds_pitch + ls;
+    }
+
+    __device__ __forceinline__ const cx<T> *post() const {
+        return reinterpret_cast<const cx<T> *>(a.tw) + a.sp.post;
+    }
+
+    // value entering the FFT at position p (stage 0)
+    __device__ __forceinline__ cx<T> load0(int p) const {
+        const T *inr = reinterpret_cast<const T *>(a.in);
+        const cx<T> *inc = reinterpret_cast<const cx<T> *>(a.in);
+        if constexpr (OP == OP_C2C) {
+            if (!valid) return cx<T>{0, 0};
+            cx<T> v = inc[ib + (int64_t)p * a.in_sm];
+            return a.inverse ? conj(v) : v;
+        } else if constexpr (OP == OP_R2C) {
+            if (!valid) return cx<T>{0, 0};
+            if constexpr (CONTIG) {
+                // adjacent reals (2p, 2p+1) of a unit-stride line: one vector load
+                return inc[(ib >> 1) + p];
+            } else {
+                return cx<T>{inr[ib + (int64_t)(2 * p) * a.in_sm], inr[ib + (int64_t)(2 * p + 1) * a.in_sm]};
+            }
+        } else if constexpr (OP == OP_R2C_FULL) {
+            if (!valid) return cx<T>{0, 0};
+            return cx<T>{inr[ib + (int64_t)p * a.in_sm], (T)0};
+        } else if constexpr (OP == OP_C2R) {
+            // the H+1 input bins sit in LDS; Z_p = A + i w^-p B (conjugated: the
+            // inverse H-point FFT runs as a forward FFT on conjugates)
+            const int H = a.sp.L;
+            cx<T> xk = lds[li(p)];
+            cx<T> xc = conj(lds[li(H - p)]);
+            cx<T> A = xk + xc, B = xk - xc;
+            cx<T> wB = cmul(conj(post()[p]), B);
+            cx<T> z = A + mul_pos_i(wB);
+            return conj(z);
+        } else if constexpr (OP == OP_C2R_FULL) {
+            if (!valid) return cx<T>{0, 0};
+            const int L = a.sp.L;
+            cx<T> x;
+            if (2 * p <= L) x = inc[ib + (int64_t)p * a.in_sm];
+            else x = conj(inc[ib + (int64_t)(L - p) * a.in_sm]);
+            if (p == 0 || 2 * p == L) x.y = 0;
+            return conj(x);
+        } else if constexpr (OP == OP_DCT2) {
+            if (!valid) return cx<T>{0, 0};
+            const int L = a.sp.L;
+            int q = (p < (L + 1) / 2) ? 2 * p : 2 * (L - 1 - p) + 1;
+            return cx<T>{inr[ib + (int64_t)q * a.in_sm], (T)0};
+        } else {  // OP_DCT3
+            if (!valid) return cx<T>{0, 0};
+            const int L = a.sp.L;
+            T xp = inr[ib + (int64_t)p * a.in_sm];
+            T xr = p == 0 ? (T)0 : inr[ib + (int64_t)(L - p) * a.in_sm];
+            // conj( exp(+i pi p / 2L) (x_p - i x_{L-p}) ) = w_p (x_p + i x_{L-p})
+            return cmul(post()[p], cx<T>{xp, xr});
+        }
+    }
+
+    // last-stage output V at natural position p
+    __device__ __forceinline__ void store_last(int p, cx<T> v) const {
+        T *outr = reinterpret_cast<T *>(a.out);
+        cx<T> *outc = reinterpret_cast<cx<T> *>(a.out);
+        const T s = (T)a.scale;
+        if constexpr (OP == OP_R2C) {
+            lds[li(p)] = v;  // post-processed after a barrier
+        } else {
+            if (!valid) return;
+            if constexpr (OP == OP_C2C) {
+                cx<T> o = a.inverse ? conj(v) : v;
+                outc[ob + (int64_t)p * a.out_sm] = scl(o, s);
+            } else if constexpr (OP == OP_R2C_FULL) {
+                if (2 * p <= a.sp.L) outc[ob + (int64_t)p * a.out_sm] = scl(v, s);
+            } else if constexpr (OP == OP_C2R) {
+                if constexpr (CONTIG) {
+                    reinterpret_cast<cx<T> *>(a.out)[(ob >> 1) + p] = cx<T>{v.x * s, -v.y * s};
+                } else {
+                    outr[ob + (int64_t)(2 * p) * a.out_sm] = v.x * s;
+                    outr[ob + (int64_t)(2 * p + 1) * a.out_sm] = -v.y * s;
+                }
+            } else if constexpr (OP == OP_C2R_FULL) {
+                outr[ob + (int64_t)p * a.out_sm] = v.x * s;
+            } else if constexpr (OP == OP_DCT2) {
+                cx<T> w = post()[p];
+                outr[ob + (int64_t)p * a.out_sm] = (T)2 * (w.x * v.x - w.y * v.y) * s;
+            } else {  // OP_DCT3
+                const int L = a.sp.L;
+                int q = (p < (L + 1) / 2) ? 2 * p : 2 * (L - 1 - p) + 1;
+                outr[ob + (int64_t)q * a.out_sm] = v.x * s;
+            }
+        }
+    }
+};
+
+// Stage position in the schedule: where its input comes from and where its
+// output goes.
+enum StageKind : int { SK_FIRST = 0, SK_MID = 1, SK_LAST = 2, SK_ONLY = 3 };
+
+// One Stockham stage with compile-time radix R.  Each thread owns up to
+// EMAX/R butterflies j = t, t+TPL, ...
+template <int R, int EMAX, int SK, class T, int OP, bool CONTIG>
+__device__ __forceinline__ void run_stage(const TileCtx<T, OP, CONTIG> &c, int Ns, int twoff) {
+    constexpr bool first = SK == SK_FIRST || SK == SK_ONLY;
+    constexpr bool last = SK == SK_LAST || SK == SK_ONLY;
+    constexpr int NBM = EMAX / R;
+    const int L = c.a.sp.L, NB = L / R, TPL = c.a.TPL;
+    cx<T> v[NBM][R];
+#pragma unroll
+    for (int b = 0; b < NBM; ++b) {
+        const int j = c.t + b * TPL;
+        if (j < NB) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if constexpr (first) v[b][r] = c.load0(j + r * NB);
+                else v[b][r] = c.lds[c.li(j + r * NB)];
+            }
+        }
+    }
+    // every read of the shared slots must finish before anyone overwrites them
+    if constexpr (!first || OP == OP_C2R) __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NBM; ++b) {
+        const int j = c.t + b * TPL;
+        if (j < NB) {
+            if constexpr (!first) {
+                const int k = j % Ns;
+                const cx<T> *w = reinterpret_cast<const cx<T> *>(c.a.tw) + twoff + k * (R - 1);
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], w[r - 1]);
+            }
+            bfly<R>(v[b]);
+            if constexpr (last) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) c.store_last(j + r * NB, v[b][r]);
+            } else {
+                const int k = first ? 0 : j % Ns;
+                const int base = (j - k) * R + k;
+#pragma unroll
+                for (int r = 0; r < R; ++r) c.lds[c.li(base + r * Ns)] = v[b][r];
+            }
+        }
+    }
+    if constexpr (!last) __syncthreads();
+}
+
+template <int SK, int EMAX, class T, int OP, bool CONTIG>
+__device__ __forceinline__ void dispatch_stage(const TileCtx<T, OP, CONTIG> &c, int R, int Ns, int twoff) {
+    switch (R) {
+    case 8: run_stage<8, EMAX, SK>(c, Ns, twoff); break;
+    case 4: run_stage<4, EMAX, SK>(c, Ns, twoff); break;
+    case 2: run_stage<2, EMAX, SK>(c, Ns, twoff); break;
+    case 3: run_stage<3, EMAX, SK>(c, Ns, twoff); break;
+    case 5: run_stage<5, EMAX, SK>(c, Ns, twoff); break;
+    case 7: run_stage<7, EMAX, SK>(c, Ns, twoff); break;
+    default: break;
+    }
+}
+
+// Runs tile `tile` of pass `a` with the calling workgroup.  Threads with
+// tid >= TPL*TC (a persistent workgroup larger than the tile needs) take part
+// in every barrier but touch no memory.
+template <int EMAX, class T, int OP, bool CONTIG>
+__device__ __forceinline__ void axis_tile(const AxisPass &a, int64_t tile, int tid, cx<T> *lds) {
+    const int TPL = a.TPL, TC = a.TC;
+    const bool active = tid < TPL * TC;
+    int ls, t;
+    if constexpr (CONTIG) {
+        ls = tid / TPL;
+        t = tid - ls * TPL;
+    } else {
+        ls = tid % TC;
+        t = tid / TC;
+    }
+    if (!active) {
+        ls = 0;
+        t = 1 << 28;  // beyond every butterfly / bin index: skips all work
+    }
+    bool valid;
+    int64_t ib, ob;
+    if constexpr (CONTIG) {
+        const int64_t line = tile * TC + ls;
+        valid = active && line < a.nlines;
+        ib = line * a.in_so;
+        ob = line * a.out_so;
+    } else {
+        const int64_t nct = (a.nlines + TC - 1) / TC;
+        const int64_t o = tile / nct;
+        const int64_t col = (tile - o * nct) * TC + ls;
+        valid = active && col < a.nlines;
+        ib = col + o * a.in_so;
+        ob = col + o * a.out_so;
+    }
+    TileCtx<T, OP, CONTIG> c{a, lds, ls, t, valid, ib, ob};
+    const StagePlan &sp = a.sp;
+
+    if constexpr (OP == OP_C2R) {
+        // stage the H+1 input bins; the DC and Nyquist imaginary parts are ignored
+        const int H = sp.L;
+        const cx<T> *inc = reinterpret_cast<const cx<T> *>(a.in);
+        for (int k = t; k <= H; k += TPL) {
+            cx<T> x = valid ? inc[ib + (int64_t)k * a.in_sm] : cx<T>{0, 0};
+            if (k == 0 || k == H) x.y = 0;
+            lds[c.li(k)] = x;
+        }
+        __syncthreads();
+    }
+
+    if (sp.S == 0) {  // length-1 transform: only the fused load/store maps apply
+        if (t == 0) c.store_last(0, c.load0(0));
+    }
+    if (sp.S == 1) {
+        dispatch_stage<SK_ONLY, EMAX>(c, sp.R[0], 1, sp.tw[0]);
+    } else if (sp.S > 1) {
+        dispatch_stage<SK_FIRST, EMAX>(c, sp.R[0], 1, sp.tw[0]);
+        int Ns = sp.R[0];
+        for (int s = 1; s < sp.S - 1; ++s) {
+            dispatch_stage<SK_MID, EMAX>(c, sp.R[s], Ns, sp.tw[s]);
+            Ns *= sp.R[s];
+        }
+        dispatch_stage<SK_LAST, EMAX>(c, sp.R[sp.S - 1], Ns, sp.tw[sp.S - 1]);
+    }
+
+    if constexpr (OP == OP_R2C) {
+        // X_k = 1/2 [(Z_k + conj Z_{H-k}) - i w^k (Z_k - conj Z_{H-k})], k = 0..H
+        __syncthreads();
+        const int H = sp.L;
+        const T s = (T)a.scale;
+        cx<T> *outc = reinterpret_cast<cx<T> *>(a.out);
+        const cx<T> *pw = c.post();
+        for (int k = t; k <= H; k += TPL) {
+            cx<T> zk = lds[c.li(k == H ? 0 : k)];
+            cx<T> zc = conj(lds[c.li(k == 0 ? 0 : H - k)]);
+            cx<T> A = zk + zc, B = zk - zc;
+            cx<T> X = scl(A + mul_neg_i(cmul(pw[k], B)), (T)0.5 * s);
+            if (valid) outc[ob + (int64_t)k * a.out_sm] = X;
+        }
+    }
+}
+
+}  // namespace dfft

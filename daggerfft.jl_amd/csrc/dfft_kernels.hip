@@ -1,0 +1,215 @@
+// dfft_kernels.hip -- HIP kernels of libdaggerfft_hip for gfx950.
+//
+//   k_axis      one Stockham axis pass per launch (C2C / R2C / C2R / DCT-II / DCT-III)
+//   k_fill      SplitMix64 counter-based uniform[0,1) input generator
+//   k_box       box gather / scatter / copy (pack_kernel!, unpack_kernel!,
+//               local_copy_kernel!; src/fftgpu.jl:452-495), coalesced along axis 0
+//   k_permute   LDS-tiled 3D axis permutation (the pencil transpose)
+#include <hip/hip_runtime.h>
+
+#include "dfft_axis.hpp"
+#include "dfft_kernels.hpp"
+
+namespace dfft {
+
+// ---------------------------------------------------------------------------
+// axis pass
+// ---------------------------------------------------------------------------
+template <class T, int OP, bool CONTIG, int EMAX>
+__global__ __launch_bounds__(1024) void k_axis(AxisPass a) {
+    extern __shared__ __align__(16) unsigned char dfft_smem[];
+    axis_tile<EMAX, T, OP, CONTIG>(a, (int64_t)blockIdx.x, (int)threadIdx.x,
+                                   reinterpret_cast<cx<T> *>(dfft_smem));
+}
+
+using axis_kernel_t = void (*)(AxisPass);
+
+#define DFFT_OPS(T, CONTIG)                                                                   \
+    {(axis_kernel_t)k_axis<T, OP_C2C, CONTIG, 8>, (axis_kernel_t)k_axis<T, OP_R2C, CONTIG, 8>, \
+     (axis_kernel_t)k_axis<T, OP_R2C_FULL, CONTIG, 8>, (axis_kernel_t)k_axis<T, OP_C2R, CONTIG, 8>, \
+     (axis_kernel_t)k_axis<T, OP_C2R_FULL, CONTIG, 8>, (axis_kernel_t)k_axis<T, OP_DCT2, CONTIG, 8>, \
+     (axis_kernel_t)k_axis<T, OP_DCT3, CONTIG, 8>}
+
+// [precision][contig][op]
+static axis_kernel_t g_axis_kernels[2][2][OP_COUNT] = {
+    {DFFT_OPS(float, false), DFFT_OPS(float, true)},
+    {DFFT_OPS(double, false), DFFT_OPS(double, true)},
+};
+
+static int g_lds_limit = 0;
+
+int axis_lds_limit() {
+    if (g_lds_limit == 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+            g_lds_limit = v;
+        else
+            g_lds_limit = 65536;
+    }
+    return g_lds_limit;
+}
+
+hipError_t launch_axis(const AxisPass &a, int prec, int64_t nblocks, int threads, size_t lds,
+                       hipStream_t stream) {
+    axis_kernel_t k = g_axis_kernels[prec ? 1 : 0][a.contig ? 1 : 0][a.op];
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k, dim3((unsigned)nblocks), dim3(threads), lds, stream, a);
+    return hipGetLastError();
+}
+
+void *axis_kernel_fn(const AxisPass &a, int prec) {
+    return (void *)g_axis_kernels[prec ? 1 : 0][a.contig ? 1 : 0][a.op];
+}
+
+// ---------------------------------------------------------------------------
+// input generator: u(seed, i) = (mix64(seed + (i+1) * golden) >> 11) * 2^-53
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double splitmix_u(uint64_t seed, uint64_t i) {
+    uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z = z ^ (z >> 31);
+    return (double)(z >> 11) * 0x1.0p-53;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_fill(T *out, int64_t nscalar, uint64_t seed) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nscalar; i += stride)
+        out[i] = (T)splitmix_u(seed, (uint64_t)i);
+}
+
+hipError_t launch_fill(void *out, int64_t nscalar, int prec, uint64_t seed, hipStream_t stream) {
+    if (nscalar <= 0) return hipSuccess;
+    int64_t blocks = (nscalar + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    if (prec) hipLaunchKernelGGL(k_fill<double>, dim3((unsigned)blocks), dim3(256), 0, stream, (double *)out, nscalar, seed);
+    else hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)blocks), dim3(256), 0, stream, (float *)out, nscalar, seed);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// box gather / scatter / copy.  Threads run along axis 0 of the box, so
+// both the strided array side and the packed buffer side are contiguous
+// bursts; blocks enumerate (i-chunk, j, k) of the box.
+//   dst[d_off + i*1 + j*d_s1 + k*d_s2] = src[s_off + i + j*s_s1 + k*s_s2]
+// ---------------------------------------------------------------------------
+struct BoxArgs {
+    const void *src;
+    void *dst;
+    int64_t s_off, s_s1, s_s2;
+    int64_t d_off, d_s1, d_s2;
+    int64_t c0, c1, c2;
+    int64_t nchunk0;
+};
+
+template <class E>
+__global__ __launch_bounds__(256) void k_box(BoxArgs b) {
+    const E *src = reinterpret_cast<const E *>(b.src);
+    E *dst = reinterpret_cast<E *>(b.dst);
+    const int64_t blk = blockIdx.x;
+    const int64_t ic = blk % b.nchunk0;
+    const int64_t jk = blk / b.nchunk0;
+    const int64_t j = jk % b.c1, k = jk / b.c1;
+    const int64_t i = ic * 256 + threadIdx.x;
+    if (i < b.c0) dst[b.d_off + i + j * b.d_s1 + k * b.d_s2] = src[b.s_off + i + j * b.s_s1 + k * b.s_s2];
+}
+
+hipError_t launch_box(const void *src, int64_t s_off, int64_t s_s1, int64_t s_s2, void *dst, int64_t d_off,
+                      int64_t d_s1, int64_t d_s2, const int64_t *cnt, int esize, hipStream_t stream) {
+    BoxArgs b{src, dst, s_off, s_s1, s_s2, d_off, d_s1, d_s2, cnt[0], cnt[1], cnt[2], (cnt[0] + 255) / 256};
+    const int64_t nblocks = b.nchunk0 * cnt[1] * cnt[2];
+    if (nblocks == 0) return hipSuccess;
+    if (nblocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    switch (esize) {
+    case 4: hipLaunchKernelGGL(k_box<uint32_t>, dim3((unsigned)nblocks), dim3(256), 0, stream, b); break;
+    case 8: hipLaunchKernelGGL(k_box<uint2>, dim3((unsigned)nblocks), dim3(256), 0, stream, b); break;
+    case 16: hipLaunchKernelGGL(k_box<uint4>, dim3((unsigned)nblocks), dim3(256), 0, stream, b); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// axis permutation through an LDS tile.  Input axis 0 (contiguous in the
+// input) and input axis `a` (contiguous in the output) span a TILE x TILE
+// tile; the remaining axis indexes the tile's plane.
+//   out[sum_d idx[d] * ost[d]] = in[idx[0] + idx[1]*n0 + idx[2]*n0*n1]
+// ---------------------------------------------------------------------------
+struct PermArgs {
+    const void *in;
+    void *out;
+    int64_t n[3];
+    int64_t ist[3];
+    int64_t ost[3];
+    int a;           // input axis that becomes output axis 0 (a != 0)
+    int b;           // remaining input axis
+    int64_t t0, ta;  // tiles along axis 0 and axis a
+};
+
+template <class E, int TILE>
+__global__ __launch_bounds__(256) void k_permute(PermArgs p) {
+    __shared__ E tile[TILE][TILE + 1];
+    const E *in = reinterpret_cast<const E *>(p.in);
+    E *out = reinterpret_cast<E *>(p.out);
+    int64_t blk = blockIdx.x;
+    const int64_t tx = blk % p.t0;
+    blk /= p.t0;
+    const int64_t ty = blk % p.ta;
+    const int64_t zb = blk / p.ta;
+    const int lx = threadIdx.x % TILE, ly = threadIdx.x / TILE;
+    constexpr int ROWS = 256 / TILE;
+    const int64_t i0 = tx * TILE, q0 = ty * TILE;
+    for (int r = ly; r < TILE; r += ROWS) {
+        const int64_t i = i0 + lx, q = q0 + r;
+        if (i < p.n[0] && q < p.n[p.a])
+            tile[r][lx] = in[i * p.ist[0] + q * p.ist[p.a] + zb * p.ist[p.b]];
+    }
+    __syncthreads();
+    for (int r = ly; r < TILE; r += ROWS) {
+        const int64_t q = q0 + lx, i = i0 + r;
+        if (i < p.n[0] && q < p.n[p.a])
+            out[i * p.ost[0] + q * p.ost[p.a] + zb * p.ost[p.b]] = tile[lx][r];
+    }
+}
+
+hipError_t launch_permute(const void *in, void *out, const int64_t *n, const int *perm, int esize,
+                          hipStream_t stream) {
+    // output strides per input axis: output axis d is input axis perm[d]
+    int64_t on[3] = {n[perm[0]], n[perm[1]], n[perm[2]]};
+    int64_t ost[3];
+    ost[perm[0]] = 1;
+    ost[perm[1]] = on[0];
+    ost[perm[2]] = on[0] * on[1];
+    int64_t ist[3] = {1, n[0], n[0] * n[1]};
+    if (perm[0] == 0) {
+        // axis 0 stays contiguous: a strided row copy, one launch per plane of the last input axis
+        int64_t cnt[3] = {n[0], n[1], n[2]};
+        return launch_box(in, 0, ist[1], ist[2], out, 0, ost[1], ost[2], cnt, esize, stream);
+    }
+    PermArgs p{};
+    p.in = in;
+    p.out = out;
+    for (int d = 0; d < 3; ++d) { p.n[d] = n[d]; p.ist[d] = ist[d]; p.ost[d] = ost[d]; }
+    p.a = perm[0];
+    p.b = 3 - p.a;  // axes {0, a, b} = {0, 1, 2}
+    const int TILE = esize == 16 ? 32 : 64;
+    p.t0 = (n[0] + TILE - 1) / TILE;
+    p.ta = (n[p.a] + TILE - 1) / TILE;
+    const int64_t nblocks = p.t0 * p.ta * n[p.b];
+    if (nblocks == 0) return hipSuccess;
+    if (nblocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    switch (esize) {
+    case 4: hipLaunchKernelGGL((k_permute<uint32_t, 64>), dim3((unsigned)nblocks), dim3(256), 0, stream, p); break;
+    case 8: hipLaunchKernelGGL((k_permute<uint2, 64>), dim3((unsigned)nblocks), dim3(256), 0, stream, p); break;
+    case 16: hipLaunchKernelGGL((k_permute<uint4, 32>), dim3((unsigned)nblocks), dim3(256), 0, stream, p); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace dfft

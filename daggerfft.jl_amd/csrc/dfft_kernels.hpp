@@ -1,0 +1,19 @@
+// dfft_kernels.hpp -- host-side launchers of the HIP kernels.
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+#include "dfft_types.hpp"
+
+namespace dfft {
+
+int axis_lds_limit();
+hipError_t launch_axis(const AxisPass &a, int prec, int64_t nblocks, int threads, size_t lds, hipStream_t stream);
+void *axis_kernel_fn(const AxisPass &a, int prec);
+hipError_t launch_fill(void *out, int64_t nscalar, int prec, uint64_t seed, hipStream_t stream);
+hipError_t launch_box(const void *src, int64_t s_off, int64_t s_s1, int64_t s_s2, void *dst, int64_t d_off,
+                      int64_t d_s1, int64_t d_s2, const int64_t *cnt, int esize, hipStream_t stream);
+hipError_t launch_permute(const void *in, void *out, const int64_t *n, const int *perm, int esize,
+                          hipStream_t stream);
+
+}  // namespace dfft

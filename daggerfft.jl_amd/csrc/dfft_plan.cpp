@@ -1,0 +1,697 @@
+// dfft_plan.cpp -- planner and C ABI of libdaggerfft_hip.
+//
+// A plan is an immutable list of axis passes.  Each pass is one kernel
+// launch over the whole array (one GPU holds the whole block, so the
+// reference's chunk stages collapse to one launch per transformed axis and
+// its redistribution collapses to strided addressing inside the pass).
+// Stage order follows the reference drivers: forward dims[0], dims[1],
+// dims[2]; inverse reversed (fft!/ifft!, src/fft.jl:651-724; Slab
+// src/fft.jl:727-776).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/daggerfft_hip.h"
+#include "dfft_kernels.hpp"
+#include "dfft_plan.hpp"
+
+namespace dfft {
+
+// ---------------------------------------------------------------------------
+// twiddles: exp(-2 pi i k / n) evaluated in long double with octant folding,
+// rounded once to the pass precision
+// ---------------------------------------------------------------------------
+void cis_ld(int64_t k, int64_t n, long double *c, long double *s) {
+    k %= n;
+    if (k < 0) k += n;
+    long double x = (long double)k / (long double)n;
+    bool neg_s = false, neg_c = false;
+    if (x > 0.5L) { x = 1.0L - x; neg_s = true; }
+    if (x > 0.25L) { x = 0.5L - x; neg_c = true; }
+    const long double TWO_PI = 6.283185307179586476925286766559005768L;
+    long double cc, ss;
+    if (x > 0.125L) {
+        long double y = (0.25L - x) * TWO_PI;
+        cc = sinl(y);
+        ss = cosl(y);
+    } else {
+        long double y = x * TWO_PI;
+        cc = cosl(y);
+        ss = sinl(y);
+    }
+    *c = neg_c ? -cc : cc;
+    *s = neg_s ? -ss : ss;
+}
+
+static void push_w(std::vector<long double> &tab, int64_t k, int64_t n) {
+    long double c, s;
+    cis_ld(k, n, &c, &s);
+    tab.push_back(c);
+    tab.push_back(-s);
+}
+
+bool factorize(int L, std::vector<int> &R) {
+    R.clear();
+    int n = L;
+    while (n % 8 == 0 && n > 0) { R.push_back(8); n /= 8; }
+    while (n % 4 == 0 && n > 0) { R.push_back(4); n /= 4; }
+    while (n % 2 == 0 && n > 0) { R.push_back(2); n /= 2; }
+    while (n % 3 == 0 && n > 0) { R.push_back(3); n /= 3; }
+    while (n % 5 == 0 && n > 0) { R.push_back(5); n /= 5; }
+    while (n % 7 == 0 && n > 0) { R.push_back(7); n /= 7; }
+    return n == 1 && (int)R.size() <= MAX_STAGES;
+}
+
+// Build the stage twiddle blocks (+ the pre/post block of fused ops) into
+// `tab` (pairs of long double); offsets are in complex elements.
+static void build_twiddles(StagePlan &sp, int op, int nlog, std::vector<long double> &tab) {
+    int Ns = 1;
+    for (int s = 0; s < sp.S; ++s) {
+        const int R = sp.R[s];
+        sp.tw[s] = (int)(tab.size() / 2);
+        if (s > 0)
+            for (int k = 0; k < Ns; ++k)
+                for (int r = 1; r < R; ++r) push_w(tab, (int64_t)k * r, (int64_t)Ns * R);
+        Ns *= R;
+    }
+    sp.post = -1;
+    if (op == OP_R2C || op == OP_C2R) {
+        // w^k = exp(-2 pi i k / (2H)), k = 0..H
+        sp.post = (int)(tab.size() / 2);
+        for (int k = 0; k <= sp.L; ++k) push_w(tab, k, 2 * (int64_t)sp.L);
+    } else if (op == OP_DCT2 || op == OP_DCT3) {
+        // exp(-i pi k / 2L), k = 0..L-1
+        sp.post = (int)(tab.size() / 2);
+        for (int k = 0; k < sp.L; ++k) push_w(tab, k, 4 * (int64_t)sp.L);
+    }
+    (void)nlog;
+}
+
+void TwiddleTable::add(AxisPass &ap) {
+    std::string key = std::to_string(ap.op) + ":" + std::to_string(ap.sp.L);
+    for (int s = 0; s < ap.sp.S; ++s) key += "," + std::to_string(ap.sp.R[s]);
+    auto it = seen.find(key);
+    if (it != seen.end()) {
+        ap.sp = it->second;
+    } else {
+        build_twiddles(ap.sp, ap.op, ap.nlog, tab);
+        seen[key] = ap.sp;
+    }
+}
+
+int TwiddleTable::upload(void **d_ptr, size_t *bytes) const {
+    const size_t es = prec ? sizeof(double) : sizeof(float);
+    const size_t nb = std::max<size_t>(tab.size() * es, 16);
+    std::vector<unsigned char> host(nb, 0);
+    if (prec) {
+        double *d = reinterpret_cast<double *>(host.data());
+        for (size_t i = 0; i < tab.size(); ++i) d[i] = (double)tab[i];
+    } else {
+        float *f = reinterpret_cast<float *>(host.data());
+        for (size_t i = 0; i < tab.size(); ++i) f[i] = (float)tab[i];
+    }
+    if (hipMalloc(d_ptr, nb) != hipSuccess) {
+        *d_ptr = nullptr;
+        return DFFT_ERR_OOM;
+    }
+    if (hipMemcpy(*d_ptr, host.data(), nb, hipMemcpyHostToDevice) != hipSuccess) return DFFT_ERR_HIP;
+    *bytes = nb;
+    return DFFT_OK;
+}
+
+static int default_tpl(int L, const std::vector<int> &R, int emax, int target_e) {
+    int tpl = std::max(1, (L + target_e - 1) / target_e);
+    for (;;) {
+        bool ok = true;
+        for (int r : R) {
+            const int nb = L / r;
+            if ((nb + tpl - 1) / tpl > emax / r) { ok = false; break; }
+        }
+        if (ok || tpl >= 1024) break;
+        ++tpl;
+    }
+    return tpl;
+}
+
+static bool tpl_ok(int L, const std::vector<int> &R, int emax, int tpl) {
+    for (int r : R)
+        if ((L / r + tpl - 1) / tpl > emax / r) return false;
+    return tpl >= 1 && tpl <= 1024;
+}
+
+// ---------------------------------------------------------------------------
+// schedule of one pass: radices + launch shape
+// ---------------------------------------------------------------------------
+int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_lds) {
+    AxisPass &a = pp.ap;
+    const int L = a.sp.L;
+    const int es = pp.prec ? 16 : 8;
+    std::vector<int> R;
+    if (tune && !tune->radices.empty()) R = tune->radices;
+    else if (!factorize(L, R)) return DFFT_ERR_UNSUPPORTED_SIZE;
+    int prod = 1;
+    for (int r : R) prod *= r;
+    if (prod != L || (int)R.size() > MAX_STAGES) return DFFT_ERR_UNSUPPORTED_SIZE;
+    a.sp.S = (int)R.size();
+    for (int s = 0; s < a.sp.S; ++s) a.sp.R[s] = R[s];
+    const int emax = EMAX;
+    int tpl = (tune && tune->tpl > 0 && tpl_ok(L, R, emax, tune->tpl)) ? tune->tpl : default_tpl(L, R, emax, 8);
+    if (L == 1) tpl = 1;
+    if (tpl > max_threads) return DFFT_ERR_UNSUPPORTED_SIZE;
+    a.TPL = tpl;
+    const int rows = L + 1;  // C2R stages H+1 bins
+    const int lds_limit = max_lds ? (int)std::min<size_t>(max_lds, (size_t)axis_lds_limit()) : axis_lds_limit();
+    if (a.contig) {
+        a.pad_shift = (tune && tune->pad_shift > 0) ? tune->pad_shift : (pp.prec ? 3 : 4);
+        a.lds_pitch = rows + (rows >> a.pad_shift) + 1;
+        int tc = (tune && tune->tc > 0) ? tune->tc : std::max(1, 256 / tpl);
+        while (tc > 1 && (int64_t)tc * tpl > max_threads) --tc;
+        while (tc > 1 && (size_t)tc * a.lds_pitch * es > (size_t)std::min(lds_limit, 65536)) --tc;
+        if ((size_t)a.lds_pitch * es > (size_t)lds_limit) return DFFT_ERR_UNSUPPORTED_SIZE;
+        a.TC = tc;
+        pp.threads = tpl * tc;
+        pp.lds = (size_t)tc * a.lds_pitch * es;
+        pp.nblocks = (a.nlines + tc - 1) / tc;
+    } else {
+        a.pad_shift = 30;
+        int tc = (tune && tune->tc > 0) ? tune->tc : (pp.prec ? 16 : 32);
+        int64_t cols = a.nlines;
+        while (tc > 1 && tc / 2 >= cols) tc /= 2;
+        while (tc > 1 && (int64_t)tc * tpl > max_threads) tc /= 2;
+        while (tc > 1 && (size_t)rows * tc * es > (size_t)lds_limit) tc /= 2;
+        if ((size_t)rows * tc * es > (size_t)lds_limit || (int64_t)tc * tpl > max_threads) return DFFT_ERR_UNSUPPORTED_SIZE;
+        a.TC = tc;
+        a.lds_pitch = tc;
+        pp.threads = tpl * tc;
+        pp.lds = (size_t)rows * tc * es;
+        pp.nblocks = ((a.nlines + tc - 1) / tc) * a.nouter;
+    }
+    if (pp.nblocks > 0x7fffffffLL) return DFFT_ERR_UNSUPPORTED_SIZE;
+    return DFFT_OK;
+}
+
+static void strides_of(const int64_t n[3], int64_t st[3]) {
+    st[0] = 1;
+    st[1] = n[0];
+    st[2] = n[0] * n[1];
+}
+
+// geometry of a pass along `axis` from array nin (input type) to nout (output type)
+static void set_geometry(AxisPass &a, int axis, const int64_t nin[3], const int64_t nout[3]) {
+    int64_t si[3], so[3];
+    strides_of(nin, si);
+    strides_of(nout, so);
+    if (si[axis] == 1) {
+        // lines contiguous and back to back
+        a.contig = 1;
+        a.nlines = (nin[0] * nin[1] * nin[2]) / nin[axis];
+        a.nouter = 1;
+        a.in_sm = 1;
+        a.in_so = nin[axis];
+        a.out_sm = 1;
+        a.out_so = nout[axis];
+    } else if (axis == 1) {
+        a.contig = 0;
+        a.nlines = nin[0];
+        a.nouter = nin[2];
+        a.in_sm = si[1];
+        a.in_so = si[2];
+        a.out_sm = so[1];
+        a.out_so = so[2];
+    } else {  // axis 2: the whole (i, j) plane is contiguous columns
+        a.contig = 0;
+        a.nlines = nin[0] * nin[1];
+        a.nouter = 1;
+        a.in_sm = si[2];
+        a.in_so = 0;
+        a.out_sm = so[2];
+        a.out_so = 0;
+    }
+}
+
+}  // namespace dfft
+
+using namespace dfft;
+
+// ---------------------------------------------------------------------------
+// plan object
+// ---------------------------------------------------------------------------
+struct dfft_plan {
+    int rank = 3;
+    int64_t n[3] = {1, 1, 1};    // logical real-space size
+    int kind = DFFT_C2C, sign = -1, prec = DFFT_F64, decomp = DFFT_PENCIL;
+    int ndims = 0;
+    int dims[3] = {0, 0, 0};
+    int r2r[3] = {0, 0, 0};
+    uint32_t flags = 0;
+    size_t in_bytes = 0, out_bytes = 0, work_bytes = 0;
+    std::vector<PassPlan> passes;
+    void *d_tw = nullptr;
+    size_t tw_bytes = 0;
+    void *own_work = nullptr;
+    int device = 0;
+    // graph replay cache
+    mutable std::mutex gmu;
+    mutable hipGraphExec_t gexec = nullptr;
+    mutable hipGraph_t graph = nullptr;
+    mutable const void *g_in = nullptr;
+    mutable void *g_out = nullptr, *g_work = nullptr;
+};
+
+static const char *op_name(int op) {
+    switch (op) {
+    case OP_C2C: return "C2C";
+    case OP_R2C: return "R2C";
+    case OP_R2C_FULL: return "R2C_FULL";
+    case OP_C2R: return "C2R";
+    case OP_C2R_FULL: return "C2R_FULL";
+    case OP_DCT2: return "DCT2";
+    case OP_DCT3: return "DCT3";
+    }
+    return "?";
+}
+
+static double r2r_logical(int kind, int64_t n) {
+    switch (kind) {
+    case DFFT_REDFT00: return 2.0 * (double)(n - 1);
+    case DFFT_RODFT00: return 2.0 * (double)(n + 1);
+    case DFFT_DHT: return (double)n;
+    default: return 2.0 * (double)n;
+    }
+}
+
+namespace dfft {
+
+// Builds passes; twiddle tables are appended to `tab` (long double pairs),
+// one block per distinct (L, op, radices) in this plan.
+int build_plan(dfft_plan *p) {
+    const int es_r = p->prec ? 8 : 4;
+    const int es_c = 2 * es_r;
+    const int64_t *n = p->n;
+    int64_t nc[3] = {n[0], n[1], n[2]};
+    const int a0 = p->dims[0];
+    if (p->kind == DFFT_R2C || p->kind == DFFT_C2R) nc[a0] = n[a0] / 2 + 1;
+    const int64_t N = n[0] * n[1] * n[2];
+    const int64_t NC = nc[0] * nc[1] * nc[2];
+    const bool inv = p->sign > 0;
+    const bool norm = !(p->flags & DFFT_FLAG_UNNORMALIZED);
+
+    std::vector<int> order;
+    for (int i = 0; i < p->ndims; ++i) order.push_back(p->dims[i]);
+
+    auto add = [&](int axis, int op, int L, int nlog, const int64_t *nin, const int64_t *nout, int src, int dst,
+                   double scale, bool inverse, double bytes) {
+        PassPlan pp{};
+        pp.prec = p->prec;
+        pp.axis = axis;
+        pp.src = src;
+        pp.dst = dst;
+        pp.alg_bytes = bytes;
+        AxisPass &a = pp.ap;
+        a.op = op;
+        a.inverse = inverse ? 1 : 0;
+        a.scale = scale;
+        a.nlog = nlog;
+        a.sp.L = L;
+        set_geometry(a, axis, nin, nout);
+        p->passes.push_back(pp);
+    };
+
+    if (p->kind == DFFT_C2C) {
+        p->in_bytes = p->out_bytes = (size_t)N * es_c;
+        if (inv) std::reverse(order.begin(), order.end());
+        for (size_t i = 0; i < order.size(); ++i) {
+            const int ax = order[i];
+            const int L = (int)n[ax];
+            add(ax, OP_C2C, L, L, n, n, i == 0 ? BUF_IN : BUF_OUT, BUF_OUT, (inv && norm) ? 1.0 / L : 1.0, inv,
+                2.0 * N * es_c);
+        }
+    } else if (p->kind == DFFT_R2C) {
+        p->in_bytes = (size_t)N * es_r;
+        p->out_bytes = (size_t)NC * es_c;
+        const int L = (int)n[a0];
+        if (L % 2 == 0) add(a0, OP_R2C, L / 2, L, n, nc, BUF_IN, BUF_OUT, 1.0, false, (double)N * es_r + (double)NC * es_c);
+        else add(a0, OP_R2C_FULL, L, L, n, nc, BUF_IN, BUF_OUT, 1.0, false, (double)N * es_r + (double)NC * es_c);
+        for (size_t i = 1; i < order.size(); ++i)
+            add(order[i], OP_C2C, (int)nc[order[i]], (int)nc[order[i]], nc, nc, BUF_OUT, BUF_OUT, 1.0, false,
+                2.0 * NC * es_c);
+    } else if (p->kind == DFFT_C2R) {
+        p->in_bytes = (size_t)NC * es_c;
+        p->out_bytes = (size_t)N * es_r;
+        const bool destroy = (p->flags & DFFT_FLAG_DESTROY_INPUT) != 0;
+        const int mid = destroy ? BUF_IN : BUF_WORK;
+        int cur = BUF_IN;
+        for (size_t i = order.size(); i-- > 1;) {
+            const int ax = order[i];
+            const int L = (int)nc[ax];
+            add(ax, OP_C2C, L, L, nc, nc, cur, mid, norm ? 1.0 / L : 1.0, true, 2.0 * NC * es_c);
+            cur = mid;
+        }
+        if (cur == BUF_WORK) p->work_bytes = (size_t)NC * es_c;
+        const int L = (int)n[a0];
+        const double sc = norm ? 1.0 / L : 1.0;
+        if (L % 2 == 0) add(a0, OP_C2R, L / 2, L, nc, n, cur, BUF_OUT, sc, true, (double)NC * es_c + (double)N * es_r);
+        else add(a0, OP_C2R_FULL, L, L, nc, n, cur, BUF_OUT, sc, true, (double)NC * es_c + (double)N * es_r);
+    } else {  // R2R
+        p->in_bytes = p->out_bytes = (size_t)N * es_r;
+        std::vector<int> kinds(p->r2r, p->r2r + p->ndims);
+        if (inv) {
+            std::reverse(order.begin(), order.end());
+            std::reverse(kinds.begin(), kinds.end());
+        }
+        for (size_t i = 0; i < order.size(); ++i) {
+            const int ax = order[i];
+            const int L = (int)n[ax];
+            int op;
+            if (kinds[i] == DFFT_REDFT10) op = OP_DCT2;
+            else if (kinds[i] == DFFT_REDFT01) op = OP_DCT3;
+            else return DFFT_ERR_UNSUPPORTED;
+            const double sc = (p->flags & DFFT_FLAG_NORMALIZE_R2R) ? 1.0 / r2r_logical(kinds[i], L) : 1.0;
+            add(ax, op, L, L, n, n, i == 0 ? BUF_IN : BUF_OUT, BUF_OUT, sc, false, 2.0 * N * es_r);
+        }
+    }
+
+    // schedules + twiddle tables (deduplicated per (op, L, radices))
+    TwiddleTable tt;
+    tt.prec = p->prec;
+    for (auto &pp : p->passes) {
+        const Tuning *tune = nullptr;
+        Tuning t;
+        if (!(p->flags & DFFT_FLAG_NO_TUNE) && tuning_lookup(pp, t)) tune = &t;
+        int rc = schedule_pass(pp, tune);
+        if (rc != DFFT_OK) return rc;
+        tt.add(pp.ap);
+    }
+    int rc = tt.upload(&p->d_tw, &p->tw_bytes);
+    if (rc != DFFT_OK) return rc;
+    for (auto &pp : p->passes) pp.ap.tw = p->d_tw;
+    return DFFT_OK;
+}
+
+}  // namespace dfft
+
+static void free_plan(dfft_plan *p) {
+    if (!p) return;
+    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->d_tw) (void)hipFree(p->d_tw);
+    if (p->own_work) (void)hipFree(p->own_work);
+    delete p;
+}
+
+static void *resolve(int buf, const void *in, void *out, void *work) {
+    switch (buf) {
+    case BUF_IN: return const_cast<void *>(in);
+    case BUF_OUT: return out;
+    default: return work;
+    }
+}
+
+static int run_passes(const dfft_plan *p, const void *d_in, void *d_out, void *work, hipStream_t st) {
+    for (const auto &pp : p->passes) {
+        AxisPass a = pp.ap;
+        a.in = resolve(pp.src, d_in, d_out, work);
+        a.out = resolve(pp.dst, d_in, d_out, work);
+        if (launch_axis(a, pp.prec, pp.nblocks, pp.threads, pp.lds, st) != hipSuccess) return DFFT_ERR_HIP;
+    }
+    return DFFT_OK;
+}
+
+static int run_graph(const dfft_plan *p, const void *d_in, void *d_out, void *work, hipStream_t st) {
+    std::lock_guard<std::mutex> g(p->gmu);
+    if (!p->gexec || p->g_in != d_in || p->g_out != d_out || p->g_work != work) {
+        if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+        if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
+        hipGraph_t graph;
+        if (hipGraphCreate(&graph, 0) != hipSuccess) return DFFT_ERR_HIP;
+        hipGraphNode_t prev = nullptr;
+        std::vector<AxisPass> args(p->passes.size());
+        for (size_t i = 0; i < p->passes.size(); ++i) {
+            const auto &pp = p->passes[i];
+            args[i] = pp.ap;
+            args[i].in = resolve(pp.src, d_in, d_out, work);
+            args[i].out = resolve(pp.dst, d_in, d_out, work);
+            if (pp.lds > 65536)
+                (void)hipFuncSetAttribute(axis_kernel_fn(args[i], pp.prec), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)pp.lds);
+            void *kargs[] = {&args[i]};
+            hipKernelNodeParams kp{};
+            kp.func = axis_kernel_fn(args[i], pp.prec);
+            kp.gridDim = dim3((unsigned)pp.nblocks);
+            kp.blockDim = dim3(pp.threads);
+            kp.sharedMemBytes = (unsigned)pp.lds;
+            kp.kernelParams = kargs;
+            hipGraphNode_t node;
+            if (hipGraphAddKernelNode(&node, graph, prev ? &prev : nullptr, prev ? 1 : 0, &kp) != hipSuccess) {
+                (void)hipGraphDestroy(graph);
+                return DFFT_ERR_HIP;
+            }
+            prev = node;
+        }
+        hipGraphExec_t exec;
+        if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            return DFFT_ERR_HIP;
+        }
+        p->graph = graph;
+        p->gexec = exec;
+        p->g_in = d_in;
+        p->g_out = d_out;
+        p->g_work = work;
+    }
+    return hipGraphLaunch(p->gexec, st) == hipSuccess ? DFFT_OK : DFFT_ERR_HIP;
+}
+
+extern "C" {
+
+int dfft_plan_create(dfft_plan **plan, int rank, const int64_t *n, int kind, int sign, const int *r2r_kinds,
+                     int precision, int decomp, const int *dims, int ndims, uint32_t flags) {
+    if (!plan || !n || !dims) return DFFT_ERR_ARG;
+    *plan = nullptr;
+    if (rank < 1 || rank > 3 || ndims < 1 || ndims > rank) return DFFT_ERR_ARG;
+    if (kind < DFFT_C2C || kind > DFFT_R2R) return DFFT_ERR_ARG;
+    if (sign != -1 && sign != 1) return DFFT_ERR_ARG;
+    if (precision != DFFT_F32 && precision != DFFT_F64) return DFFT_ERR_ARG;
+    if (decomp != DFFT_PENCIL && decomp != DFFT_SLAB) return DFFT_ERR_ARG;
+    if (kind == DFFT_R2C && sign != -1) return DFFT_ERR_ARG;
+    if (kind == DFFT_C2R && sign != 1) return DFFT_ERR_ARG;
+    if (kind == DFFT_R2R && !r2r_kinds) return DFFT_ERR_ARG;
+    dfft_plan *p = nullptr;
+    try {
+        p = new dfft_plan();
+    } catch (...) {
+        return DFFT_ERR_OOM;
+    }
+    p->rank = rank;
+    for (int d = 0; d < rank; ++d) {
+        if (n[d] < 1 || n[d] > (1LL << 31)) { delete p; return DFFT_ERR_ARG; }
+        p->n[d] = n[d];
+    }
+    p->kind = kind;
+    p->sign = sign;
+    p->prec = precision;
+    p->decomp = decomp;
+    p->flags = flags;
+    p->ndims = ndims;
+    bool used[3] = {false, false, false};
+    for (int i = 0; i < ndims; ++i) {
+        if (dims[i] < 0 || dims[i] >= rank || used[dims[i]]) { delete p; return DFFT_ERR_ARG; }
+        used[dims[i]] = true;
+        p->dims[i] = dims[i];
+        if (kind == DFFT_R2R) {
+            if (r2r_kinds[i] < DFFT_R2HC || r2r_kinds[i] > DFFT_RODFT11) { delete p; return DFFT_ERR_ARG; }
+            p->r2r[i] = r2r_kinds[i];
+        }
+    }
+    const int64_t total = p->n[0] * p->n[1] * p->n[2];
+    if (total > (1LL << 40)) { delete p; return DFFT_ERR_ARG; }
+    if (hipGetDevice(&p->device) != hipSuccess) { delete p; return DFFT_ERR_HIP; }
+    int rc;
+    try {
+        rc = build_plan(p);
+    } catch (const std::bad_alloc &) {
+        rc = DFFT_ERR_OOM;
+    } catch (...) {
+        rc = DFFT_ERR_ARG;
+    }
+    if (rc != DFFT_OK) {
+        free_plan(p);
+        return rc;
+    }
+    *plan = p;
+    return DFFT_OK;
+}
+
+int dfft_plan_workspace_size(const dfft_plan *p, size_t *bytes) {
+    if (!p || !bytes) return DFFT_ERR_ARG;
+    *bytes = p->work_bytes;
+    return DFFT_OK;
+}
+
+int dfft_plan_io_size(const dfft_plan *p, size_t *in_bytes, size_t *out_bytes) {
+    if (!p) return DFFT_ERR_ARG;
+    if (in_bytes) *in_bytes = p->in_bytes;
+    if (out_bytes) *out_bytes = p->out_bytes;
+    return DFFT_OK;
+}
+
+int dfft_execute(const dfft_plan *p, const void *d_in, void *d_out, void *d_work, void *stream) {
+    if (!p || !d_in || !d_out) return DFFT_ERR_ARG;
+    if (d_in == d_out && (p->kind == DFFT_R2C || p->kind == DFFT_C2R)) return DFFT_ERR_ARG;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    void *work = d_work;
+    if (p->work_bytes && !work) {
+        dfft_plan *mp = const_cast<dfft_plan *>(p);
+        std::lock_guard<std::mutex> g(mp->gmu);
+        if (!mp->own_work && hipMalloc(&mp->own_work, p->work_bytes) != hipSuccess) {
+            mp->own_work = nullptr;
+            return DFFT_ERR_OOM;
+        }
+        work = mp->own_work;
+    }
+    if (p->flags & DFFT_FLAG_GRAPH) return run_graph(p, d_in, d_out, work, st);
+    return run_passes(p, d_in, d_out, work, st);
+}
+
+int dfft_execute_pass(const dfft_plan *p, int pass, const void *d_in, void *d_out, void *d_work, void *stream) {
+    if (!p || !d_in || !d_out || pass < 0 || pass >= (int)p->passes.size()) return DFFT_ERR_ARG;
+    if (d_in == d_out && (p->kind == DFFT_R2C || p->kind == DFFT_C2R)) return DFFT_ERR_ARG;
+    void *work = d_work;
+    if (p->work_bytes && !work) {
+        dfft_plan *mp = const_cast<dfft_plan *>(p);
+        std::lock_guard<std::mutex> g(mp->gmu);
+        if (!mp->own_work && hipMalloc(&mp->own_work, p->work_bytes) != hipSuccess) {
+            mp->own_work = nullptr;
+            return DFFT_ERR_OOM;
+        }
+        work = mp->own_work;
+    }
+    const PassPlan &pp = p->passes[pass];
+    AxisPass a = pp.ap;
+    a.in = resolve(pp.src, d_in, d_out, work);
+    a.out = resolve(pp.dst, d_in, d_out, work);
+    return launch_axis(a, pp.prec, pp.nblocks, pp.threads, pp.lds, (hipStream_t)stream) == hipSuccess ? DFFT_OK
+                                                                                                     : DFFT_ERR_HIP;
+}
+
+int dfft_plan_destroy(dfft_plan *p) {
+    if (!p) return DFFT_ERR_ARG;
+    free_plan(p);
+    return DFFT_OK;
+}
+
+int dfft_plan_num_passes(const dfft_plan *p, int *npasses) {
+    if (!p || !npasses) return DFFT_ERR_ARG;
+    *npasses = (int)p->passes.size();
+    return DFFT_OK;
+}
+
+int dfft_plan_describe(const dfft_plan *p, int pass, char *buf, size_t buflen) {
+    if (!p || !buf || pass < 0 || pass >= (int)p->passes.size()) return DFFT_ERR_ARG;
+    const PassPlan &pp = p->passes[pass];
+    const AxisPass &a = pp.ap;
+    std::string rad;
+    for (int s = 0; s < a.sp.S; ++s) rad += (s ? "x" : "") + std::to_string(a.sp.R[s]);
+    if (rad.empty()) rad = "1";
+    snprintf(buf, buflen,
+             "{\"pass\":%d,\"axis\":%d,\"op\":\"%s\",\"L\":%d,\"nlog\":%d,\"radices\":\"%s\",\"TPL\":%d,\"TC\":%d,"
+             "\"contig\":%d,\"threads\":%d,\"blocks\":%lld,\"lds\":%zu,\"bytes\":%.0f,\"prec\":\"%s\"}",
+             pass, pp.axis, op_name(a.op), a.sp.L, a.nlog, rad.c_str(), a.TPL, a.TC, a.contig, pp.threads,
+             (long long)pp.nblocks, pp.lds, pp.alg_bytes, pp.prec ? "f64" : "f32");
+    return DFFT_OK;
+}
+
+int dfft_plan_algorithmic_bytes(const dfft_plan *p, double *bytes) {
+    if (!p || !bytes) return DFFT_ERR_ARG;
+    double b = 0;
+    for (const auto &pp : p->passes) b += pp.alg_bytes;
+    *bytes = b;
+    return DFFT_OK;
+}
+
+const char *dfft_strerror(int s) {
+    switch (s) {
+    case DFFT_OK: return "success";
+    case DFFT_ERR_ARG: return "invalid argument";
+    case DFFT_ERR_UNSUPPORTED_SIZE: return "unsupported transform length (prime factor > 7 or too long)";
+    case DFFT_ERR_HIP: return "HIP runtime error";
+    case DFFT_ERR_OOM: return "device out of memory";
+    case DFFT_ERR_BUDGET: return "request exceeds the HBM budget";
+    case DFFT_ERR_STATE: return "object used in the wrong state";
+    case DFFT_ERR_NOT_READY: return "request not finished";
+    case DFFT_ERR_UNSUPPORTED: return "transform kind not implemented on the GPU path";
+    }
+    return "unknown status";
+}
+
+int dfft_version(int *major, int *minor) {
+    if (major) *major = 0;
+    if (minor) *minor = 1;
+    return DFFT_OK;
+}
+
+static bool esize_ok(int e) { return e == 4 || e == 8 || e == 16; }
+
+int dfft_box_pack(void *d_buf, const void *d_src, const int64_t *src_shape, const int64_t *lo, const int64_t *cnt,
+                  int64_t offset, int esize, void *stream) {
+    if (!d_buf || !d_src || !src_shape || !lo || !cnt || !esize_ok(esize) || offset < 0) return DFFT_ERR_ARG;
+    for (int d = 0; d < 3; ++d)
+        if (lo[d] < 0 || cnt[d] < 0 || lo[d] + cnt[d] > src_shape[d]) return DFFT_ERR_ARG;
+    const int64_t s1 = src_shape[0], s2 = src_shape[0] * src_shape[1];
+    hipError_t e = launch_box(d_src, lo[0] + lo[1] * s1 + lo[2] * s2, s1, s2, d_buf, offset, cnt[0], cnt[0] * cnt[1],
+                              cnt, esize, (hipStream_t)stream);
+    return e == hipSuccess ? DFFT_OK : DFFT_ERR_HIP;
+}
+
+int dfft_box_unpack(void *d_dst, const void *d_buf, const int64_t *dst_shape, const int64_t *lo, const int64_t *cnt,
+                    int64_t offset, int esize, void *stream) {
+    if (!d_buf || !d_dst || !dst_shape || !lo || !cnt || !esize_ok(esize) || offset < 0) return DFFT_ERR_ARG;
+    for (int d = 0; d < 3; ++d)
+        if (lo[d] < 0 || cnt[d] < 0 || lo[d] + cnt[d] > dst_shape[d]) return DFFT_ERR_ARG;
+    const int64_t s1 = dst_shape[0], s2 = dst_shape[0] * dst_shape[1];
+    hipError_t e = launch_box(d_buf, offset, cnt[0], cnt[0] * cnt[1], d_dst, lo[0] + lo[1] * s1 + lo[2] * s2, s1, s2,
+                              cnt, esize, (hipStream_t)stream);
+    return e == hipSuccess ? DFFT_OK : DFFT_ERR_HIP;
+}
+
+int dfft_box_copy(void *d_dst, const int64_t *dst_shape, const int64_t *dlo, const void *d_src,
+                  const int64_t *src_shape, const int64_t *slo, const int64_t *cnt, int esize, void *stream) {
+    if (!d_dst || !d_src || !dst_shape || !src_shape || !dlo || !slo || !cnt || !esize_ok(esize)) return DFFT_ERR_ARG;
+    for (int d = 0; d < 3; ++d) {
+        if (cnt[d] < 0 || dlo[d] < 0 || slo[d] < 0) return DFFT_ERR_ARG;
+        if (dlo[d] + cnt[d] > dst_shape[d] || slo[d] + cnt[d] > src_shape[d]) return DFFT_ERR_ARG;
+    }
+    const int64_t s1 = src_shape[0], s2 = src_shape[0] * src_shape[1];
+    const int64_t d1 = dst_shape[0], d2 = dst_shape[0] * dst_shape[1];
+    hipError_t e = launch_box(d_src, slo[0] + slo[1] * s1 + slo[2] * s2, s1, s2, d_dst,
+                              dlo[0] + dlo[1] * d1 + dlo[2] * d2, d1, d2, cnt, esize, (hipStream_t)stream);
+    return e == hipSuccess ? DFFT_OK : DFFT_ERR_HIP;
+}
+
+int dfft_permute(void *d_out, const void *d_in, const int64_t *in_shape, const int *perm, int esize, void *stream) {
+    if (!d_out || !d_in || !in_shape || !perm || !esize_ok(esize)) return DFFT_ERR_ARG;
+    bool seen[3] = {false, false, false};
+    for (int d = 0; d < 3; ++d) {
+        if (perm[d] < 0 || perm[d] > 2 || seen[perm[d]] || in_shape[d] < 1) return DFFT_ERR_ARG;
+        seen[perm[d]] = true;
+    }
+    if (d_out == d_in) return DFFT_ERR_ARG;
+    hipError_t e = launch_permute(d_in, d_out, in_shape, perm, esize, (hipStream_t)stream);
+    return e == hipSuccess ? DFFT_OK : DFFT_ERR_HIP;
+}
+
+int dfft_fill_uniform(void *d_out, int64_t nelem, int is_complex, int precision, uint64_t seed, void *stream) {
+    if (!d_out || nelem < 0 || (precision != DFFT_F32 && precision != DFFT_F64)) return DFFT_ERR_ARG;
+    hipError_t e = launch_fill(d_out, is_complex ? 2 * nelem : nelem, precision, seed, (hipStream_t)stream);
+    return e == hipSuccess ? DFFT_OK : DFFT_ERR_HIP;
+}
+
+}  // extern "C"

@@ -1,0 +1,54 @@
+// dfft_plan.hpp -- host-side pass descriptors shared by the planner, the
+// batched launcher and the autotuner.
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "dfft_types.hpp"
+
+namespace dfft {
+
+constexpr int EMAX = 8;  // complex elements a thread may hold per stage
+
+enum BufId : int { BUF_IN = 0, BUF_OUT = 1, BUF_WORK = 2 };
+
+struct PassPlan {
+    AxisPass ap{};
+    int prec = 1;
+    int axis = 0;
+    int src = BUF_IN, dst = BUF_OUT;
+    int64_t nblocks = 0;
+    int threads = 0;
+    size_t lds = 0;
+    double alg_bytes = 0;
+};
+
+// A schedule override from the autotuner cache.
+struct Tuning {
+    std::vector<int> radices;
+    int tpl = 0;
+    int tc = 0;
+    int pad_shift = 0;
+};
+
+bool factorize(int L, std::vector<int> &R);
+int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads = 1024, size_t max_lds = 0);
+
+// Stage twiddle + pre/post blocks of every distinct (op, L, radices) of one
+// precision, in one device table.
+struct TwiddleTable {
+    int prec = 1;
+    std::vector<long double> tab;           // (re, im) pairs
+    std::map<std::string, StagePlan> seen;
+    void add(AxisPass &ap);                 // sets ap.sp.tw / ap.sp.post offsets
+    int upload(void **d_ptr, size_t *bytes) const;
+};
+bool tuning_lookup(const PassPlan &pp, Tuning &t);
+std::string tuning_key(const PassPlan &pp);
+void cis_ld(int64_t k, int64_t n, long double *c, long double *s);
+
+}  // namespace dfft

@@ -1,0 +1,54 @@
+// dfft_types.hpp -- plain-old-data descriptors shared by the host planner and
+// the HIP kernels.  One AxisPass describes one kernel launch: a batch of 1D
+// transforms of length L along one axis of a column-major array.
+#pragma once
+#include <stdint.h>
+
+namespace dfft {
+
+constexpr int MAX_STAGES = 14;
+
+// Operation fused into an axis pass.
+enum PassOp : int {
+    OP_C2C = 0,       // complex -> complex (sign via `inverse`)
+    OP_R2C = 1,       // real line of length 2H -> H+1 bins via an H-point complex FFT + post-twiddle
+    OP_R2C_FULL = 2,  // real line of length L (odd) -> L/2+1 bins via an L-point complex FFT
+    OP_C2R = 3,       // H+1 bins -> real line of length 2H via pre-twiddle + H-point inverse FFT
+    OP_C2R_FULL = 4,  // L/2+1 bins -> real line of length L (odd) via an L-point inverse FFT
+    OP_DCT2 = 5,      // REDFT10 via an L-point FFT of the even/odd reordered line (Makhoul 1980)
+    OP_DCT3 = 6,      // REDFT01 via a pre-twiddled L-point inverse FFT (Makhoul 1980)
+    OP_COUNT = 7
+};
+
+// Stockham schedule of one line length.
+struct StagePlan {
+    int L;                    // complex FFT length
+    int S;                    // number of stages
+    int R[MAX_STAGES];        // radix of each stage (2,3,4,5,7,8)
+    int tw[MAX_STAGES];       // offset (complex elements) of each stage's twiddle block
+    int post;                 // offset of the pre/post-twiddle block (R2C/C2R/DCT), -1 if none
+};
+
+// Geometry + schedule of one launch.  Element strides are counted in
+// elements of the array's own type (real or complex).
+struct AxisPass {
+    const void *in;
+    void *out;
+    const void *tw;           // twiddle table of the pass precision (complex)
+    StagePlan sp;
+    int op;
+    int inverse;              // 1: backward transform (conjugate on load/store)
+    double scale;             // applied on store
+    int contig;               // 1: each line is contiguous (axis 0); 0: columns contiguous
+    int TPL;                  // threads per line
+    int TC;                   // lines (contig) / columns (strided) per workgroup
+    int lds_pitch;            // contig: elements per line slot; strided: elements per row
+    int pad_shift;            // contig: slot index p -> p + (p >> pad_shift)
+    int nlog;                 // logical line length (real length for R2C/C2R/DCT)
+    int64_t nlines;           // contig: number of lines; strided: number of columns
+    int64_t nouter;           // strided: number of outer slices (1 for contig)
+    int64_t in_sm, in_so;     // input: stride along the line, stride between lines/outer
+    int64_t out_sm, out_so;   // output: same
+};
+
+}  // namespace dfft

@@ -1,0 +1,207 @@
+/*
+ * daggerfft_hip.h -- C ABI of libdaggerfft_hip, the MI355X (gfx950) 3D FFT
+ * library behind the DaggerFFT.jl transform path.
+ *
+ * Every entry point returns an int status (DFFT_OK == 0); nothing throws or
+ * aborts across this boundary.  All device work is enqueued on the caller's
+ * HIP stream (passed as void*; NULL = the default stream) and nothing
+ * synchronises the device implicitly except where stated.
+ *
+ * Layout: arrays are column-major (Julia order).  An array of size
+ * n[0] x n[1] x n[2] holds element (i,j,k) at i + n[0]*(j + n[1]*k); complex
+ * elements are interleaved (re, im).  Axes are 0-based here (Julia's dims are
+ * 1-based; the Julia shim subtracts 1).
+ *
+ * What each entry point replaces in the reference (Julia, sanatgp/DaggerFFT.jl):
+ *   dfft_plan_create     plan_transform / gpu_plan_transform + the plan caches
+ *                        get_or_create_plan / get_or_create_gpu_plan
+ *                        (src/fft.jl:126-166, src/fftgpu.jl:83-116), and the
+ *                        per-stage structure of fft!/ifft!/gpu_fft!/gpu_ifft!
+ *                        for Pencil and Slab (src/fft.jl:650-776,
+ *                        src/fftgpu.jl:497-619)
+ *   dfft_execute         the three-stage driver bodies: apply_fft! /
+ *                        apply_gpu_fft! (`mul!(out, plan, in)`, src/fft.jl:168-174,
+ *                        src/fftgpu.jl:118-124) per stage plus the on-device
+ *                        LOCAL_COPIES redistribution (src/fftgpu.jl:367-385)
+ *   dfft_plan_destroy    cleanup_workspace! / gpu_cleanup_workspace!
+ *                        (src/fft.jl:778-781, src/fftgpu.jl:630-634)
+ *   dfft_box_*           pack_kernel!, unpack_kernel!, local_copy_kernel!
+ *                        (src/fftgpu.jl:452-495) and pack_data!/unpack_data!/
+ *                        local_copy! (src/fft.jl:180-231)
+ *   dfft_batch_*         no reference counterpart: many independent transforms
+ *                        in one persistent launch
+ *   dfft_server_*        no reference counterpart: continuous-batching request loop
+ */
+#ifndef DAGGERFFT_HIP_H
+#define DAGGERFFT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define DFFT_OK                    0
+#define DFFT_ERR_ARG               1  /* bad argument (ArgumentError in the reference) */
+#define DFFT_ERR_UNSUPPORTED_SIZE  2  /* a length with a prime factor > 7, or too long */
+#define DFFT_ERR_HIP               3  /* a HIP runtime call failed */
+#define DFFT_ERR_OOM               4  /* device allocation failed */
+#define DFFT_ERR_BUDGET            5  /* serving admission: request exceeds the HBM budget */
+#define DFFT_ERR_STATE             6  /* object used in the wrong state */
+#define DFFT_ERR_NOT_READY         7  /* serving: request not finished yet */
+#define DFFT_ERR_UNSUPPORTED       8  /* transform kind not implemented on the GPU path */
+
+/* ---- transform kinds (FFT / RFFT / IRFFT tags, src/fft.jl:17-24; R2R README.md:59-66) */
+#define DFFT_C2C  0
+#define DFFT_R2C  1
+#define DFFT_C2R  2
+#define DFFT_R2R  3
+
+/* ---- r2r kinds: FFTW's fftw_r2r_kind numbering (R2R_SUPPORTED_KINDS, src/fft.jl:1-11) */
+#define DFFT_R2HC     0
+#define DFFT_HC2R     1
+#define DFFT_DHT      2
+#define DFFT_REDFT00  3
+#define DFFT_REDFT01  4
+#define DFFT_REDFT10  5
+#define DFFT_REDFT11  6
+#define DFFT_RODFT00  7
+#define DFFT_RODFT01  8
+#define DFFT_RODFT10  9
+#define DFFT_RODFT11 10
+
+/* ---- precision --------------------------------------------------------- */
+#define DFFT_F32  0   /* Float32 / ComplexF32 data, twiddles generated in FP64+ and rounded once */
+#define DFFT_F64  1   /* Float64 / ComplexF64 */
+
+/* ---- decomposition (Pencil / Slab, src/fft.jl:13-15) ------------------- */
+#define DFFT_PENCIL 0
+#define DFFT_SLAB   1
+
+/* ---- plan flags -------------------------------------------------------- */
+#define DFFT_FLAG_UNNORMALIZED   0x1u  /* sign=+1 C2C/C2R without the 1/n scaling (FFTW bfft) */
+#define DFFT_FLAG_NORMALIZE_R2R  0x2u  /* scale R2R by 1/prod(logical size) (README ifft with REDFT01) */
+#define DFFT_FLAG_DESTROY_INPUT  0x4u  /* C2R may overwrite its input (no workspace needed) */
+#define DFFT_FLAG_NO_TUNE        0x8u  /* skip the autotuner cache; use the heuristic schedule */
+#define DFFT_FLAG_GRAPH          0x10u /* replay the pass list as a hipGraph */
+
+typedef struct dfft_plan dfft_plan;
+
+/*
+ * Create a plan.  Replaces plan_transform (src/fft.jl:126-144) /
+ * gpu_plan_transform (src/fftgpu.jl:85-105) and the Pencil/Slab stage lists
+ * of fft!/ifft! (src/fft.jl:651-776).
+ *   rank      1..3
+ *   n         logical (real-space) size per axis, n[0] fastest
+ *   kind      DFFT_C2C / R2C / C2R / R2R
+ *   sign      -1 forward, +1 inverse (C2C: ifft, scaled 1/n per dim unless
+ *             DFFT_FLAG_UNNORMALIZED).  R2C requires -1, C2R requires +1.
+ *             For R2R the sign only sets the stage order.
+ *   r2r_kinds one FFTW kind per entry of dims (R2R only; else may be NULL)
+ *   precision DFFT_F32 / DFFT_F64
+ *   decomp    DFFT_PENCIL / DFFT_SLAB
+ *   dims      0-based axes to transform, in stage order (ndims 1..3, distinct).
+ *             R2C/C2R halve dims[0] to n/2+1 (FFTW rfft convention).
+ */
+int dfft_plan_create(dfft_plan **plan, int rank, const int64_t *n, int kind, int sign,
+                     const int *r2r_kinds, int precision, int decomp,
+                     const int *dims, int ndims, uint32_t flags);
+
+/* Bytes of device workspace dfft_execute needs when d_work != NULL (0 if none). */
+int dfft_plan_workspace_size(const dfft_plan *plan, size_t *bytes);
+
+/* Bytes of the input and output arrays for this plan. */
+int dfft_plan_io_size(const dfft_plan *plan, size_t *in_bytes, size_t *out_bytes);
+
+/*
+ * Execute on `stream`.  Replaces `mul!(out, plan, in)` per stage plus the
+ * single-device redistribution (src/fft.jl:170, src/fftgpu.jl:120,367-385).
+ * d_in == d_out runs in place (C2C and R2R only).  d_work may be NULL: the
+ * plan then uses (and owns) its own workspace.  Concurrent executions of one
+ * plan need distinct d_work buffers.
+ */
+int dfft_execute(const dfft_plan *plan, const void *d_in, void *d_out, void *d_work, void *stream);
+
+int dfft_plan_destroy(dfft_plan *plan);
+
+/* Execute one axis pass of the plan (0 <= pass < num_passes), so a caller can
+ * time or profile each axis pass on its own.  Running passes 0..n-1 in order
+ * on one stream is exactly dfft_execute.  No reference counterpart (the
+ * reference times whole stages with NVTX ranges, src/fft.jl:654-686). */
+int dfft_execute_pass(const dfft_plan *plan, int pass, const void *d_in, void *d_out, void *d_work,
+                      void *stream);
+
+/* Number of axis passes and their names/geometry, for profiling and tests. */
+int dfft_plan_num_passes(const dfft_plan *plan, int *npasses);
+int dfft_plan_describe(const dfft_plan *plan, int pass, char *buf, size_t buflen);
+
+/* Algorithmic HBM bytes of the plan: sum over passes of bytes read + written. */
+int dfft_plan_algorithmic_bytes(const dfft_plan *plan, double *bytes);
+
+const char *dfft_strerror(int status);
+int dfft_version(int *major, int *minor);
+
+/*
+ * Box maps of the redistribution, on device.  `esize` is the element size in
+ * bytes (4, 8 or 16).  Offsets, lo and cnt are 0-based element counts.
+ *   dfft_box_pack:   buf[offset + i + c0*(j + c1*k)] = src[lo + (i,j,k)]
+ *                    (pack_kernel!, src/fftgpu.jl:452-465; pack_data!, src/fft.jl:180-194)
+ *   dfft_box_unpack: dst[lo + (i,j,k)] = buf[offset + i + c0*(j + c1*k)]
+ *                    (unpack_kernel!, src/fftgpu.jl:467-480; unpack_data!, src/fft.jl:196-214)
+ *   dfft_box_copy:   dst[dlo + (i,j,k)] = src[slo + (i,j,k)]
+ *                    (local_copy_kernel!, src/fftgpu.jl:482-495; local_copy!, src/fft.jl:216-231)
+ *   dfft_permute:    out = in with axes permuted: out axis d is in axis perm[d]
+ *                    (the pencil transpose as an LDS-tiled axis transpose)
+ */
+int dfft_box_pack(void *d_buf, const void *d_src, const int64_t *src_shape, const int64_t *lo,
+                  const int64_t *cnt, int64_t offset, int esize, void *stream);
+int dfft_box_unpack(void *d_dst, const void *d_buf, const int64_t *dst_shape, const int64_t *lo,
+                    const int64_t *cnt, int64_t offset, int esize, void *stream);
+int dfft_box_copy(void *d_dst, const int64_t *dst_shape, const int64_t *dlo, const void *d_src,
+                  const int64_t *src_shape, const int64_t *slo, const int64_t *cnt, int esize,
+                  void *stream);
+int dfft_permute(void *d_out, const void *d_in, const int64_t *in_shape, const int *perm, int esize,
+                 void *stream);
+
+/* Seeded SplitMix64 uniform[0,1) fill on device (the benchmark input generator):
+ * real element e <- u(seed, e); complex element e <- (u(seed, 2e), u(seed, 2e+1)). */
+int dfft_fill_uniform(void *d_out, int64_t nelem, int is_complex, int precision, uint64_t seed,
+                      void *stream);
+
+/* ---- batched persistent launch ------------------------------------------
+ * No reference counterpart: many independent transforms in one persistent
+ * launch (the reference runs one cuFFT plan per chunk, src/fftgpu.jl:118-124).
+ * Each transform is rank 3 over dims (0,1,2); C2C (either sign) or R2C
+ * (forward, even n[0]).  Results equal the same transform run alone.      */
+typedef struct {
+    int64_t n[3];       /* size of this 3D transform (n[0] fastest) */
+    int kind;           /* DFFT_C2C or DFFT_R2C */
+    int sign;           /* -1 / +1 (C2C); -1 (R2C) */
+    int precision;      /* DFFT_F32 / DFFT_F64 */
+    uint32_t flags;     /* DFFT_FLAG_UNNORMALIZED */
+} dfft_batch_desc;
+
+typedef struct dfft_batch dfft_batch;
+
+int dfft_batch_create(dfft_batch **batch, int count, const dfft_batch_desc *descs, uint32_t flags);
+/* d_in[t], d_out[t]: device pointers of transform t (in place when equal, C2C only).
+ * Synchronises `stream` first (the item table is staged through pinned memory). */
+int dfft_batch_execute(dfft_batch *batch, void *const *d_in, void *const *d_out, void *stream);
+int dfft_batch_workspace_size(const dfft_batch *batch, size_t *bytes);
+/* total number of tiles (work tickets) of one execution */
+int dfft_batch_num_items(const dfft_batch *batch, int64_t *items);
+int dfft_batch_algorithmic_bytes(const dfft_batch *batch, double *bytes);
+/* Synchronises `stream` and reports whether a dependency wait hit the spin
+ * limit during the last execution (1) or not (0). */
+int dfft_batch_error(dfft_batch *batch, void *stream, int *flag);
+/* Bound on one dependency wait, in polling rounds (tests force tiny limits). */
+int dfft_batch_set_spin_limit(dfft_batch *batch, uint64_t limit);
+int dfft_batch_destroy(dfft_batch *batch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DAGGERFFT_HIP_H */

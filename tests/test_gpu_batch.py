@@ -1,0 +1,90 @@
+"""Batched persistent launch (64 mixed-size 3D transforms in one launch):
+every result equals the same transform planned and run alone, and the
+oracle within tolerance (ComplexF64 rel-L2 <= 1e-12; ComplexF32 vs the FP64
+oracle <= 1e-5)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_batch(descs, seeds):
+    import daggerfft as D
+    from daggerfft.batch import Batch
+    b = Batch(descs)
+    ins, outs = [], []
+    for (shape, kind, sign, prec), sd in zip(descs, seeds):
+        if kind == 0:
+            dt = np.complex128 if prec == 1 else np.complex64
+            ins.append(D.rand(shape, dt, seed=sd))
+            outs.append(D.DeviceArray(shape, dt))
+        else:
+            dt = np.float64 if prec == 1 else np.float32
+            ins.append(D.rand(shape, dt, seed=sd))
+            oshape = (shape[0] // 2 + 1,) + tuple(shape[1:])
+            outs.append(D.DeviceArray(oshape, np.complex128 if prec == 1 else np.complex64))
+    b.execute(ins, outs)
+    assert b.error_flag() == 0
+    return b, ins, outs
+
+
+def test_batch_config_64_mixed(gpu):
+    import daggerfft as D
+    from daggerfft import _lib as L
+    from daggerfft.batch import batch_config_sizes
+    cfg = batch_config_sizes(64, 7)
+    descs = [(shape, L.C2C, -1, prec) for shape, prec in cfg]
+    seeds = [1000 + t for t in range(64)]
+    b, ins, outs = _run_batch(descs, seeds)
+    worst64 = worst32 = 0.0
+    for t, ((shape, prec), X, Y) in enumerate(zip(cfg, ins, outs)):
+        alone = D.fft(X)
+        yb = Y.to_host()
+        assert np.array_equal(yb, alone.to_host()) or O.rel_l2(yb, alone.to_host()) < 1e-15, t
+        if t % 8 == 0 or t < 4:  # oracle on a subset keeps the CPU time low
+            dt = np.complex128 if prec == 1 else np.complex64
+            a = O.fill(1000 + t, shape, dt).astype(np.complex128)
+            e = O.rel_l2(yb.astype(np.complex128), O.fftn(a))
+            if prec == 1:
+                worst64 = max(worst64, e)
+            else:
+                worst32 = max(worst32, e)
+    print("batch of 64: worst rel-L2 f64 %.3e, f32 %.3e (vs FP64 oracle), tickets %d" % (worst64, worst32, b.tickets))
+    assert worst64 <= 1e-12 and worst32 <= 1e-5
+
+
+def test_batch_inverse_and_r2c_mix(gpu):
+    import daggerfft as D
+    from daggerfft import _lib as L
+    descs = [((32, 48, 20), L.C2C, +1, 1), ((64, 30, 14), L.R2C, -1, 1), ((16, 16, 16), L.C2C, -1, 0),
+             ((40, 21, 10), L.R2C, -1, 0), ((7, 5, 3), L.C2C, -1, 1)]
+    seeds = [51, 52, 53, 54, 55]
+    b, ins, outs = _run_batch(descs, seeds)
+    for (shape, kind, sign, prec), X, Y in zip(descs, ins, outs):
+        if kind == L.C2C:
+            alone = D.fft(X) if sign < 0 else D.ifft(X)
+        else:
+            alone = D.rfft(X)
+        tol = 1e-15 if prec == 1 else 1e-7
+        assert O.rel_l2(Y.to_host(), alone.to_host()) <= tol
+
+
+def test_batch_spin_limit_reports_error(gpu):
+    """A tiny spin limit forced from the host makes dependency waits give up:
+    the kernel still drains and the error flag is raised."""
+    import daggerfft as D
+    from daggerfft import _lib as L
+    from daggerfft.batch import Batch
+    descs = [((256, 256, 64), L.C2C, -1, 1)] * 2
+    b = Batch(descs)
+    b.set_spin_limit(1)
+    ins = [D.rand(d[0], np.complex128, seed=3) for d in descs]
+    outs = [D.DeviceArray(d[0], np.complex128) for d in descs]
+    b.execute(ins, outs)
+    assert b.error_flag() == 1
+    b.set_spin_limit(1 << 32)
+    b.execute(ins, outs)
+    assert b.error_flag() == 0
+    assert O.rel_l2(outs[0].to_host(), D.fft(ins[0]).to_host()) < 1e-15

@@ -1,0 +1,128 @@
+"""R2C / C2R (rfft / irfft, README.md:38-44) and R2R DCT-II / DCT-III
+(REDFT10 / REDFT01, README.md:59-66) on the HIP path vs the oracle.
+FP64 tolerance: relative L2 <= 1e-12."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+TOL64 = 1e-12
+
+
+def _dev(a):
+    import daggerfft as D
+    return D.DeviceArray.from_host(a)
+
+
+@pytest.mark.parametrize("shape", [(16, 12, 10), (15, 6, 4), (2, 3, 3), (1, 5, 4), (64, 64, 64), (30, 7, 9)])
+@pytest.mark.parametrize("decomp", ["pencil", "slab"])
+def test_rfft_irfft_vs_oracle(gpu, shape, decomp):
+    import daggerfft as D
+    dec = D.Pencil() if decomp == "pencil" else D.Slab()
+    a = O.fill(2, shape, np.float64)
+    F = D.rfft(_dev(a), decomp=dec)
+    ref = O.rfftn(a)
+    assert F.shape == ref.shape
+    e = O.rel_l2(F.to_host(), ref)
+    print(shape, decomp, "rfft rel-L2", e)
+    assert e <= TOL64
+    B = D.irfft(F, shape[0], decomp=dec)
+    assert B.shape == shape
+    assert O.rel_l2(B.to_host(), O.irfftn(ref, shape[0])) <= TOL64
+    assert O.rel_l2(B.to_host(), a) <= TOL64
+
+
+@pytest.mark.parametrize("dims", [(2,), (3,), (2, 1), (3, 2, 1), (2, 3)])
+def test_rfft_first_dim_strided(gpu, dims):
+    import daggerfft as D
+    shape = (12, 10, 14)
+    a = O.fill(4, shape, np.float64)
+    F = D.rfft(_dev(a), dims=dims)
+    ref = O.rfftn(a, dims=dims)
+    assert O.rel_l2(F.to_host(), ref) <= TOL64
+    d = shape[dims[0] - 1]
+    B = D.irfft(F, d, dims=dims)
+    assert O.rel_l2(B.to_host(), a) <= TOL64
+
+
+def test_rfft_256_cubed(gpu):
+    import daggerfft as D
+    shape = (256, 256, 256)
+    A = D.rand(shape, np.float64, seed=2)
+    a = O.fill(2, shape, np.float64)
+    F = D.rfft(A)
+    ref = O.rfftn(a)
+    e = O.rel_l2(F.to_host(), ref)
+    print("256^3 rfft rel-L2", e)
+    assert e <= TOL64
+    B = D.irfft(F, 256)
+    e2 = O.rel_l2(B.to_host(), a)
+    print("256^3 irfft round trip rel-L2", e2)
+    assert e2 <= TOL64
+
+
+def test_irfft_ignores_dc_nyquist_imag(gpu):
+    import daggerfft as D
+    shape = (8, 6)
+    a = O.fill(3, shape, np.float64)
+    ref = O.rfftn(a)
+    bad = ref.copy()
+    bad[0, :] += 0.25j
+    bad[4, :] -= 0.5j
+    B = D.irfft(_dev(np.asfortranarray(bad)), 8)
+    assert O.rel_l2(B.to_host(), O.irfftn(bad, 8)) <= TOL64
+
+
+def test_rfft_f32(gpu):
+    import daggerfft as D
+    shape = (64, 48, 30)
+    A = D.rand(shape, np.float32, seed=12)
+    a = O.fill(12, shape, np.float32).astype(np.float64)
+    F = D.rfft(A)
+    e = O.rel_l2(F.to_host().astype(np.complex128), O.rfftn(a))
+    print("f32 rfft rel-L2 vs f64 oracle", e)
+    assert e <= 1e-5
+    B = D.irfft(F, 64)
+    assert O.rel_l2(B.to_host().astype(np.float64), a) <= 1e-5
+
+
+@pytest.mark.parametrize("shape,dims", [((256, 256), (1, 2)), ((128, 128, 128), (1, 2, 3)), ((7, 5, 3), (1, 2, 3)),
+                                        ((1, 9, 4), (1, 2, 3)), ((30, 21, 10), (2, 3))])
+def test_dct2_dct3_vs_oracle(gpu, shape, dims):
+    import daggerfft as D
+    a = O.fill(4, shape, np.float64)
+    kinds = (D.REDFT10,) * len(dims)
+    F = D.fft(_dev(a), decomp=D.Slab(), transforms=(D.R2R(kinds),), dims=dims)
+    ref = O.r2rn(a, O.REDFT10, dims)
+    e = O.rel_l2(F.to_host(), ref)
+    print(shape, "REDFT10 rel-L2", e)
+    assert e <= TOL64
+    # FFTW-faithful REDFT01 (unnormalised) on the GPU vs the oracle
+    from daggerfft import _lib as L
+    p = D.Plan(shape, L.R2R, -1, L.F64, D.Slab(), dims, r2r_kinds=(D.REDFT01,) * len(dims))
+    G = p.execute(F, F.similar())
+    ref3 = O.r2rn(ref, O.REDFT01, dims)
+    assert O.rel_l2(G.to_host(), ref3) <= TOL64
+    # README ifft with REDFT01 reconstructs A (scaled by 1/prod(2n))
+    B = D.ifft(F, decomp=D.Slab(), transforms=(D.R2R((D.REDFT01,) * len(dims)),), dims=dims)
+    assert O.rel_l2(B.to_host(), a) <= TOL64
+
+
+def test_mixed_r2r_kinds_per_dim(gpu):
+    import daggerfft as D
+    from daggerfft import _lib as L
+    shape = (12, 10, 6)
+    a = O.fill(8, shape, np.float64)
+    p = D.Plan(shape, L.R2R, -1, L.F64, D.Pencil(), (1, 2, 3), r2r_kinds=(D.REDFT10, D.REDFT01, D.REDFT10))
+    G = p.execute(_dev(a), D.DeviceArray(shape, np.float64))
+    ref = O.r2rn(a, (O.REDFT10, O.REDFT01, O.REDFT10), (1, 2, 3))
+    assert O.rel_l2(G.to_host(), ref) <= TOL64
+
+
+def test_unsupported_r2r_kind_raises(gpu):
+    import daggerfft as D
+    from daggerfft import _lib as L
+    with pytest.raises(D.DFFTError) as ei:
+        D.Plan((8, 8), L.R2R, -1, L.F64, D.Pencil(), (1, 2), r2r_kinds=(D.DHT, D.DHT))
+    assert ei.value.status == L.DFFT_ERR_UNSUPPORTED
