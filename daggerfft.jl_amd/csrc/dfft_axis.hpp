@@ -29,6 +29,8 @@ namespace dfft {
 template <class T, int OP, bool CONTIG>
 struct TileCtx {
     const AxisPass &a;
+    const void *in;
+    void *out;
     cx<T> *lds;
     int ls, t;
     bool valid;
@@ -45,8 +47,8 @@ struct TileCtx {
 
     // value entering the FFT at position p (stage 0)
     __device__ __forceinline__ cx<T> load0(int p) const {
-        const T *inr = reinterpret_cast<const T *>(a.in);
-        const cx<T> *inc = reinterpret_cast<const cx<T> *>(a.in);
+        const T *inr = reinterpret_cast<const T *>(in);
+        const cx<T> *inc = reinterpret_cast<const cx<T> *>(in);
         if constexpr (OP == OP_C2C) {
             if (!valid) return cx<T>{0, 0};
             cx<T> v = inc[ib + (int64_t)p * a.in_sm];
@@ -97,8 +99,8 @@ struct TileCtx {
 
     // last-stage output V at natural position p
     __device__ __forceinline__ void store_last(int p, cx<T> v) const {
-        T *outr = reinterpret_cast<T *>(a.out);
-        cx<T> *outc = reinterpret_cast<cx<T> *>(a.out);
+        T *outr = reinterpret_cast<T *>(out);
+        cx<T> *outc = reinterpret_cast<cx<T> *>(out);
         const T s = (T)a.scale;
         if constexpr (OP == OP_R2C) {
             lds[li(p)] = v;  // post-processed after a barrier
@@ -111,7 +113,7 @@ struct TileCtx {
                 if (2 * p <= a.sp.L) outc[ob + (int64_t)p * a.out_sm] = scl(v, s);
             } else if constexpr (OP == OP_C2R) {
                 if constexpr (CONTIG) {
-                    reinterpret_cast<cx<T> *>(a.out)[(ob >> 1) + p] = cx<T>{v.x * s, -v.y * s};
+                    reinterpret_cast<cx<T> *>(out)[(ob >> 1) + p] = cx<T>{v.x * s, -v.y * s};
                 } else {
                     outr[ob + (int64_t)(2 * p) * a.out_sm] = v.x * s;
                     outr[ob + (int64_t)(2 * p + 1) * a.out_sm] = -v.y * s;
@@ -198,7 +200,8 @@ __device__ __forceinline__ void dispatch_stage(const TileCtx<T, OP, CONTIG> &c, 
 // tid >= TPL*TC (a persistent workgroup larger than the tile needs) take part
 // in every barrier but touch no memory.
 template <int EMAX, class T, int OP, bool CONTIG>
-__device__ __forceinline__ void axis_tile(const AxisPass &a, int64_t tile, int tid, cx<T> *lds) {
+__device__ __forceinline__ void axis_tile(const AxisPass &a, const void *in, void *out, int64_t tile, int tid,
+                                          cx<T> *lds) {
     const int TPL = a.TPL, TC = a.TC;
     const bool active = tid < TPL * TC;
     int ls, t;
@@ -228,13 +231,13 @@ __device__ __forceinline__ void axis_tile(const AxisPass &a, int64_t tile, int t
         ib = col + o * a.in_so;
         ob = col + o * a.out_so;
     }
-    TileCtx<T, OP, CONTIG> c{a, lds, ls, t, valid, ib, ob};
+    TileCtx<T, OP, CONTIG> c{a, in, out, lds, ls, t, valid, ib, ob};
     const StagePlan &sp = a.sp;
 
     if constexpr (OP == OP_C2R) {
         // stage the H+1 input bins; the DC and Nyquist imaginary parts are ignored
         const int H = sp.L;
-        const cx<T> *inc = reinterpret_cast<const cx<T> *>(a.in);
+        const cx<T> *inc = reinterpret_cast<const cx<T> *>(in);
         for (int k = t; k <= H; k += TPL) {
             cx<T> x = valid ? inc[ib + (int64_t)k * a.in_sm] : cx<T>{0, 0};
             if (k == 0 || k == H) x.y = 0;
@@ -263,7 +266,7 @@ __device__ __forceinline__ void axis_tile(const AxisPass &a, int64_t tile, int t
         __syncthreads();
         const int H = sp.L;
         const T s = (T)a.scale;
-        cx<T> *outc = reinterpret_cast<cx<T> *>(a.out);
+        cx<T> *outc = reinterpret_cast<cx<T> *>(out);
         const cx<T> *pw = c.post();
         for (int k = t; k <= H; k += TPL) {
             cx<T> zk = lds[c.li(k == H ? 0 : k)];

@@ -1,0 +1,379 @@
+// dfft_group.hip -- grouped launches: tiles of many independent axis passes
+// in one kernel launch.
+//
+// A group is a sequence of launches on one stream.  Each launch holds a list
+// of items (an item = one axis pass over a slice of two pointer slots); its
+// workgroups are dealt out to the items' tiles through a block -> item map,
+// exactly as the single-pass kernel deals them to one pass.  Items inside a
+// launch are independent; every dependency between items is a launch
+// boundary, so no workgroup ever waits for another.
+//
+// Users:
+//   * dfft_batch_*: launch l runs pass l of every transform of the batch;
+//   * Slab plans: the 2D stage is software-pipelined over chunks of z-planes:
+//     launch i runs the dims[0] pass of chunk i together with the dims[1]
+//     pass of chunk i-1, which the previous launch just wrote, so that chunk
+//     is read back while it is still in the 256 MB memory-side Infinity
+//     Cache.  The dims[2] pass follows as one ordinary launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <numeric>
+#include <vector>
+
+#include "../../include/daggerfft_hip.h"
+#include "dfft_axis.hpp"
+#include "dfft_group.hpp"
+#include "dfft_kernels.hpp"
+#include "dfft_plan.hpp"
+
+namespace dfft {
+
+struct GroupArgs {
+    const GroupItem *items;     // items of this launch
+    const int *block_item;      // block -> item (index into `items`)
+    void *slots[GROUP_MAX_SLOTS];
+};
+
+template <class T>
+__device__ __forceinline__ void group_tile(const AxisPass &a, const void *in, void *out, int64_t tile, int tid,
+                                           void *smem) {
+    cx<T> *lds = reinterpret_cast<cx<T> *>(smem);
+    switch (a.op) {
+    case OP_C2C:
+        if (a.contig) axis_tile<EMAX, T, OP_C2C, true>(a, in, out, tile, tid, lds);
+        else axis_tile<EMAX, T, OP_C2C, false>(a, in, out, tile, tid, lds);
+        break;
+    case OP_R2C:
+        if (a.contig) axis_tile<EMAX, T, OP_R2C, true>(a, in, out, tile, tid, lds);
+        else axis_tile<EMAX, T, OP_R2C, false>(a, in, out, tile, tid, lds);
+        break;
+    case OP_C2R:
+        if (a.contig) axis_tile<EMAX, T, OP_C2R, true>(a, in, out, tile, tid, lds);
+        else axis_tile<EMAX, T, OP_C2R, false>(a, in, out, tile, tid, lds);
+        break;
+    default:
+        break;
+    }
+}
+
+template <int THREADS>
+__global__ __launch_bounds__(THREADS) void k_group(GroupArgs g) {
+    extern __shared__ __align__(16) unsigned char dfft_smem[];
+    const int item = __builtin_amdgcn_readfirstlane(g.block_item[blockIdx.x]);
+    const GroupItem &it = g.items[item];
+    const void *in = reinterpret_cast<const unsigned char *>(g.slots[it.in_slot]) + it.in_off;
+    void *out = reinterpret_cast<unsigned char *>(g.slots[it.out_slot]) + it.out_off;
+    const int64_t tile = (int64_t)blockIdx.x - it.first_block;
+    if (it.prec) group_tile<double>(it.ap, in, out, tile, (int)threadIdx.x, dfft_smem);
+    else group_tile<float>(it.ap, in, out, tile, (int)threadIdx.x, dfft_smem);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct GroupLaunch {
+    int first_item = 0, nitems = 0;
+    int64_t nblocks = 0;
+    int threads = 256;
+    size_t lds = 0;
+    int64_t map_off = 0;  // offset of this launch's block -> item map
+};
+
+struct GroupExec {
+    std::vector<GroupItem> items;
+    std::vector<GroupLaunch> launches;
+    GroupItem *d_items = nullptr;
+    int *d_map = nullptr;
+};
+
+void group_destroy(GroupExec *g) {
+    if (!g) return;
+    if (g->d_items) (void)hipFree(g->d_items);
+    if (g->d_map) (void)hipFree(g->d_map);
+    delete g;
+}
+
+static const void *group_kernel(int threads) {
+    switch (threads) {
+    case 1024: return (const void *)k_group<1024>;
+    case 512: return (const void *)k_group<512>;
+    default: return (const void *)k_group<256>;
+    }
+}
+
+int group_create(GroupExec **out, const std::vector<std::vector<GroupItem>> &levels) {
+    *out = nullptr;
+    GroupExec *g = new (std::nothrow) GroupExec();
+    if (!g) return DFFT_ERR_OOM;
+    std::vector<int> map;
+    const int lim = axis_lds_limit();
+    for (const auto &lv : levels) {
+        if (lv.empty()) continue;
+        GroupLaunch L;
+        L.first_item = (int)g->items.size();
+        L.nitems = (int)lv.size();
+        L.map_off = (int64_t)map.size();
+        int need = 1;
+        for (size_t i = 0; i < lv.size(); ++i) {
+            GroupItem it = lv[i];
+            if (it.in_slot < 0 || it.in_slot >= GROUP_MAX_SLOTS || it.out_slot < 0 || it.out_slot >= GROUP_MAX_SLOTS ||
+                it.ntiles < 1 || it.threads < 1 || it.threads > 1024) {
+                group_destroy(g);
+                return DFFT_ERR_ARG;
+            }
+            it.first_block = L.nblocks;
+            for (int64_t b = 0; b < it.ntiles; ++b) map.push_back((int)i);
+            L.nblocks += it.ntiles;
+            need = std::max(need, it.threads);
+            L.lds = std::max(L.lds, it.lds);
+            g->items.push_back(it);
+        }
+        L.threads = need <= 256 ? 256 : need <= 512 ? 512 : 1024;
+        if (L.lds > (size_t)lim || L.nblocks > 0x7fffffffLL) {
+            group_destroy(g);
+            return DFFT_ERR_UNSUPPORTED_SIZE;
+        }
+        g->launches.push_back(L);
+    }
+    if (g->items.empty()) {
+        group_destroy(g);
+        return DFFT_ERR_ARG;
+    }
+    if (hipMalloc(&g->d_items, sizeof(GroupItem) * g->items.size()) != hipSuccess ||
+        hipMalloc(&g->d_map, sizeof(int) * map.size()) != hipSuccess) {
+        group_destroy(g);
+        return DFFT_ERR_OOM;
+    }
+    if (hipMemcpy(g->d_items, g->items.data(), sizeof(GroupItem) * g->items.size(), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(g->d_map, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        group_destroy(g);
+        return DFFT_ERR_HIP;
+    }
+    for (const auto &L : g->launches)
+        if (L.lds > 65536 && hipFuncSetAttribute(group_kernel(L.threads), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 lim) != hipSuccess) {
+            group_destroy(g);
+            return DFFT_ERR_HIP;
+        }
+    *out = g;
+    return DFFT_OK;
+}
+
+int group_launch(const GroupExec *g, void *const *slots, int nslots, hipStream_t st) {
+    if (nslots > GROUP_MAX_SLOTS) return DFFT_ERR_ARG;
+    GroupArgs a{};
+    for (int i = 0; i < nslots; ++i) a.slots[i] = slots[i];
+    for (const auto &L : g->launches) {
+        a.items = g->d_items + L.first_item;
+        a.block_item = g->d_map + L.map_off;
+        if (L.threads == 1024)
+            hipLaunchKernelGGL(k_group<1024>, dim3((unsigned)L.nblocks), dim3(1024), L.lds, st, a);
+        else if (L.threads == 512)
+            hipLaunchKernelGGL(k_group<512>, dim3((unsigned)L.nblocks), dim3(512), L.lds, st, a);
+        else
+            hipLaunchKernelGGL(k_group<256>, dim3((unsigned)L.nblocks), dim3(256), L.lds, st, a);
+        if (hipGetLastError() != hipSuccess) return DFFT_ERR_HIP;
+    }
+    return DFFT_OK;
+}
+
+int group_num_launches(const GroupExec *g) { return (int)g->launches.size(); }
+
+int64_t group_blocks(const GroupExec *g) {
+    int64_t s = 0;
+    for (const auto &L : g->launches) s += L.nblocks;
+    return s;
+}
+
+GroupItem make_group_item(const PassPlan &pp, int in_slot, int64_t in_off, int out_slot, int64_t out_off) {
+    GroupItem it{};
+    it.ap = pp.ap;
+    it.ap.in = nullptr;
+    it.ap.out = nullptr;
+    it.prec = pp.prec;
+    it.in_slot = in_slot;
+    it.out_slot = out_slot;
+    it.in_off = in_off;
+    it.out_off = out_off;
+    it.ntiles = pp.nblocks;
+    it.threads = pp.threads;
+    it.lds = pp.lds;
+    return it;
+}
+
+}  // namespace dfft
+
+using namespace dfft;
+
+// ---------------------------------------------------------------------------
+// batch API: launch l runs pass l of every transform
+// ---------------------------------------------------------------------------
+struct dfft_batch {
+    int count = 0;
+    std::vector<dfft_batch_desc> descs;
+    GroupExec *g = nullptr;
+    void *d_tw64 = nullptr, *d_tw32 = nullptr;
+    size_t tw64_bytes = 0, tw32_bytes = 0;
+    double alg_bytes = 0;
+};
+
+static void batch_free(dfft_batch *b) {
+    if (!b) return;
+    group_destroy(b->g);
+    if (b->d_tw64) (void)hipFree(b->d_tw64);
+    if (b->d_tw32) (void)hipFree(b->d_tw32);
+    delete b;
+}
+
+extern "C" {
+
+int dfft_batch_create(dfft_batch **batch, int count, const dfft_batch_desc *descs, uint32_t flags) {
+    if (!batch || count < 1 || !descs) return DFFT_ERR_ARG;
+    *batch = nullptr;
+    if (2 * count > GROUP_MAX_SLOTS) return DFFT_ERR_ARG;
+    for (int t = 0; t < count; ++t) {
+        const dfft_batch_desc &d = descs[t];
+        if (d.kind != DFFT_C2C && d.kind != DFFT_R2C) return DFFT_ERR_UNSUPPORTED;
+        if (d.sign != -1 && d.sign != 1) return DFFT_ERR_ARG;
+        if (d.kind == DFFT_R2C && (d.sign != -1 || d.n[0] % 2 != 0)) return DFFT_ERR_ARG;
+        if (d.precision != DFFT_F32 && d.precision != DFFT_F64) return DFFT_ERR_ARG;
+        for (int a = 0; a < 3; ++a)
+            if (d.n[a] < 1 || d.n[a] > (1 << 20)) return DFFT_ERR_ARG;
+    }
+    dfft_batch *b = new (std::nothrow) dfft_batch();
+    if (!b) return DFFT_ERR_OOM;
+    b->count = count;
+    b->descs.assign(descs, descs + count);
+    TwiddleTable tt64, tt32;
+    tt64.prec = 1;
+    tt32.prec = 0;
+    std::vector<std::vector<PassPlan>> lv(3);
+    std::vector<std::vector<int>> lv_t(3);
+    int rc = DFFT_OK;
+    for (int t = 0; t < count && rc == DFFT_OK; ++t) {
+        const dfft_batch_desc &d = descs[t];
+        const int prec = d.precision;
+        const int es_c = prec ? 16 : 8;
+        const bool inv = d.sign > 0;
+        const bool norm = !(d.flags & DFFT_FLAG_UNNORMALIZED);
+        const int64_t n[3] = {d.n[0], d.n[1], d.n[2]};
+        int64_t nc[3] = {n[0], n[1], n[2]};
+        std::vector<int> order = {0, 1, 2};
+        if (d.kind == DFFT_R2C) nc[0] = n[0] / 2 + 1;
+        else if (inv) std::reverse(order.begin(), order.end());
+        const double N = (double)(n[0] * n[1] * n[2]), NC = (double)(nc[0] * nc[1] * nc[2]);
+        for (int i = 0; i < 3; ++i) {
+            const int ax = order[i];
+            PassPlan pp{};
+            pp.prec = prec;
+            pp.axis = ax;
+            pp.src = i == 0 ? BUF_IN : BUF_OUT;
+            pp.dst = BUF_OUT;
+            AxisPass &a = pp.ap;
+            if (d.kind == DFFT_R2C && i == 0) {
+                a.op = OP_R2C;
+                a.sp.L = (int)(n[0] / 2);
+                a.nlog = (int)n[0];
+                a.scale = 1.0;
+                set_geometry(a, 0, n, nc);
+                pp.alg_bytes = N * es_c / 2 + NC * es_c;
+            } else {
+                const int64_t *gsz = d.kind == DFFT_R2C ? nc : n;
+                const int L = (int)gsz[ax];
+                a.op = OP_C2C;
+                a.sp.L = L;
+                a.nlog = L;
+                a.inverse = inv ? 1 : 0;
+                a.scale = (inv && norm) ? 1.0 / L : 1.0;
+                set_geometry(a, ax, gsz, gsz);
+                pp.alg_bytes = 2.0 * (d.kind == DFFT_R2C ? NC : N) * es_c;
+            }
+            b->alg_bytes += pp.alg_bytes;
+            Tuning tn;
+            const Tuning *tune = nullptr;
+            if (!(flags & DFFT_FLAG_NO_TUNE) && tuning_lookup(pp, tn) && tn.tpl <= 512) tune = &tn;
+            rc = schedule_pass(pp, tune, 512, 65536);
+            if (rc != DFFT_OK) break;
+            (prec ? tt64 : tt32).add(pp.ap);
+            lv[i].push_back(pp);
+            lv_t[i].push_back(t);
+        }
+    }
+    if (rc == DFFT_OK && !tt64.tab.empty()) rc = tt64.upload(&b->d_tw64, &b->tw64_bytes);
+    if (rc == DFFT_OK && !tt32.tab.empty()) rc = tt32.upload(&b->d_tw32, &b->tw32_bytes);
+    if (rc != DFFT_OK) {
+        batch_free(b);
+        return rc;
+    }
+    std::vector<std::vector<GroupItem>> levels(3);
+    for (int l = 0; l < 3; ++l) {
+        // larger passes first: their tiles start earliest
+        std::vector<int> idx(lv[l].size());
+        std::iota(idx.begin(), idx.end(), 0);
+        std::stable_sort(idx.begin(), idx.end(),
+                         [&](int x, int y) { return lv[l][x].alg_bytes > lv[l][y].alg_bytes; });
+        for (int k : idx) {
+            PassPlan pp = lv[l][k];
+            const int t = lv_t[l][k];
+            pp.ap.tw = pp.prec ? b->d_tw64 : b->d_tw32;
+            const int in_slot = pp.src == BUF_IN ? 2 * t : 2 * t + 1;
+            levels[l].push_back(make_group_item(pp, in_slot, 0, 2 * t + 1, 0));
+        }
+    }
+    rc = group_create(&b->g, levels);
+    if (rc != DFFT_OK) {
+        batch_free(b);
+        return rc;
+    }
+    *batch = b;
+    return DFFT_OK;
+}
+
+int dfft_batch_execute(dfft_batch *b, void *const *d_in, void *const *d_out, void *stream) {
+    if (!b || !d_in || !d_out) return DFFT_ERR_ARG;
+    std::vector<void *> slots(2 * b->count);
+    for (int t = 0; t < b->count; ++t) {
+        if (!d_in[t] || !d_out[t]) return DFFT_ERR_ARG;
+        if (d_in[t] == d_out[t] && b->descs[t].kind != DFFT_C2C) return DFFT_ERR_ARG;
+        slots[2 * t] = d_in[t];
+        slots[2 * t + 1] = d_out[t];
+    }
+    return group_launch(b->g, slots.data(), (int)slots.size(), (hipStream_t)stream);
+}
+
+int dfft_batch_num_items(const dfft_batch *b, int64_t *items) {
+    if (!b || !items) return DFFT_ERR_ARG;
+    *items = group_blocks(b->g);
+    return DFFT_OK;
+}
+
+int dfft_batch_num_launches(const dfft_batch *b, int *launches) {
+    if (!b || !launches) return DFFT_ERR_ARG;
+    *launches = group_num_launches(b->g);
+    return DFFT_OK;
+}
+
+int dfft_batch_workspace_size(const dfft_batch *b, size_t *bytes) {
+    if (!b || !bytes) return DFFT_ERR_ARG;
+    *bytes = 0;
+    return DFFT_OK;
+}
+
+int dfft_batch_algorithmic_bytes(const dfft_batch *b, double *bytes) {
+    if (!b || !bytes) return DFFT_ERR_ARG;
+    *bytes = b->alg_bytes;
+    return DFFT_OK;
+}
+
+int dfft_batch_destroy(dfft_batch *b) {
+    if (!b) return DFFT_ERR_ARG;
+    batch_free(b);
+    return DFFT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,38 @@
+// dfft_group.hpp -- grouped launches of independent axis passes (see dfft_group.hip).
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "dfft_plan.hpp"
+#include "dfft_types.hpp"
+
+namespace dfft {
+
+constexpr int GROUP_MAX_SLOTS = 256;
+
+// One axis pass over a slice of two pointer slots, cut into `ntiles` tiles.
+struct GroupItem {
+    AxisPass ap;              // geometry + schedule; ap.in/ap.out unused
+    int prec;
+    int in_slot, out_slot;
+    int64_t in_off, out_off;  // byte offsets into the slots
+    int64_t ntiles;
+    int64_t first_block;      // first block of this item in its launch
+    int threads;              // threads a tile needs (<= the launch's workgroup size)
+    size_t lds;               // tile LDS bytes
+};
+
+struct GroupExec;
+
+// levels[l] = the items of launch l (independent of each other)
+int group_create(GroupExec **out, const std::vector<std::vector<GroupItem>> &levels);
+int group_launch(const GroupExec *g, void *const *slots, int nslots, hipStream_t st);
+void group_destroy(GroupExec *g);
+int group_num_launches(const GroupExec *g);
+int64_t group_blocks(const GroupExec *g);
+GroupItem make_group_item(const PassPlan &pp, int in_slot, int64_t in_off, int out_slot, int64_t out_off);
+
+}  // namespace dfft

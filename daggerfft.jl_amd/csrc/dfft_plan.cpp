@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -20,6 +21,7 @@
 #include <vector>
 
 #include "../../include/daggerfft_hip.h"
+#include "dfft_group.hpp"
 #include "dfft_kernels.hpp"
 #include "dfft_plan.hpp"
 
@@ -150,7 +152,7 @@ static bool tpl_ok(int L, const std::vector<int> &R, int emax, int tpl) {
 // ---------------------------------------------------------------------------
 // schedule of one pass: radices + launch shape
 // ---------------------------------------------------------------------------
-int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_lds) {
+int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_lds, int contig_threads) {
     AxisPass &a = pp.ap;
     const int L = a.sp.L;
     const int es = pp.prec ? 16 : 8;
@@ -172,9 +174,10 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
     if (a.contig) {
         a.pad_shift = (tune && tune->pad_shift > 0) ? tune->pad_shift : (pp.prec ? 3 : 4);
         a.lds_pitch = rows + (rows >> a.pad_shift) + 1;
-        int tc = (tune && tune->tc > 0) ? tune->tc : std::max(1, 256 / tpl);
+        int tc = (tune && tune->tc > 0) ? tune->tc : std::max(1, contig_threads / tpl);
         while (tc > 1 && (int64_t)tc * tpl > max_threads) --tc;
-        while (tc > 1 && (size_t)tc * a.lds_pitch * es > (size_t)std::min(lds_limit, 65536)) --tc;
+        const int contig_lds = contig_threads > 256 ? lds_limit : std::min(lds_limit, 65536);
+        while (tc > 1 && (size_t)tc * a.lds_pitch * es > (size_t)contig_lds) --tc;
         if ((size_t)a.lds_pitch * es > (size_t)lds_limit) return DFFT_ERR_UNSUPPORTED_SIZE;
         a.TC = tc;
         pp.threads = tpl * tc;
@@ -205,7 +208,7 @@ static void strides_of(const int64_t n[3], int64_t st[3]) {
 }
 
 // geometry of a pass along `axis` from array nin (input type) to nout (output type)
-static void set_geometry(AxisPass &a, int axis, const int64_t nin[3], const int64_t nout[3]) {
+void set_geometry(AxisPass &a, int axis, const int64_t nin[3], const int64_t nout[3]) {
     int64_t si[3], so[3];
     strides_of(nin, si);
     strides_of(nout, so);
@@ -264,6 +267,8 @@ struct dfft_plan {
     mutable hipGraph_t graph = nullptr;
     mutable const void *g_in = nullptr;
     mutable void *g_out = nullptr, *g_work = nullptr;
+    // Slab: plane-pipelined 2D stage + 1D stage in one persistent launch
+    GroupExec *slab = nullptr;  // Slab software pipeline (null: one launch per pass)
 };
 
 static const char *op_name(int op) {
@@ -390,18 +395,105 @@ int build_plan(dfft_plan *p) {
         if (rc != DFFT_OK) return rc;
         tt.add(pp.ap);
     }
+    // Slab: the 2D stage (dims[0], dims[1]) is software-pipelined over chunks
+    // of z-planes; the dims[2] pass follows as one launch
+    const bool slab_ok = p->decomp == DFFT_SLAB && p->rank == 3 && p->ndims == 3 && n[2] > 1 &&
+                         p->dims[2] == 2 && p->kind != DFFT_R2R && !(p->flags & DFFT_FLAG_GRAPH) &&
+                         std::all_of(p->passes.begin(), p->passes.end(), [](const PassPlan &q) {
+                             // ops the grouped kernel dispatches
+                             return q.ap.op == OP_C2C || q.ap.op == OP_R2C || q.ap.op == OP_C2R;
+                         });
+    // chunk: planes whose dims[0]-pass output is about `chunk_bytes`
+    int64_t chunk_bytes = 32LL << 20;
+    if (const char *e = getenv("DFFT_SLAB_CHUNK_BYTES")) chunk_bytes = std::max(1LL, atoll(e));
+    struct ChunkPass {
+        PassPlan pp;
+        int64_t k0, planes;
+    };
+    std::vector<ChunkPass> cpa, cpb;  // chunk passes of the first / second 2D-stage pass
+    if (slab_ok) {
+        const bool inv = p->sign > 0;
+        const int ia = inv ? 1 : 0, ib = inv ? 2 : 1;
+        auto sizes = [&](const PassPlan &q, int64_t *nin, int64_t *nout) {
+            const bool in_real = q.ap.op == OP_R2C || q.ap.op == OP_R2C_FULL;
+            const bool out_real = q.ap.op == OP_C2R || q.ap.op == OP_C2R_FULL;
+            for (int d = 0; d < 3; ++d) {
+                nin[d] = in_real ? n[d] : nc[d];
+                nout[d] = out_real ? n[d] : nc[d];
+            }
+        };
+        int64_t ain[3], aout[3];
+        sizes(p->passes[ia], ain, aout);
+        const int64_t plane_out = aout[0] * aout[1] * ((p->passes[ia].ap.op == OP_C2R) ? es_r : es_c);
+        const int64_t K = std::max<int64_t>(1, std::min<int64_t>(n[2], chunk_bytes / std::max<int64_t>(1, plane_out)));
+        for (int which = 0; which < 2; ++which) {
+            const PassPlan &full = p->passes[which == 0 ? ia : ib];
+            int64_t nin[3], nout[3];
+            sizes(full, nin, nout);
+            for (int64_t k0 = 0; k0 < n[2]; k0 += K) {
+                const int64_t planes = std::min(K, n[2] - k0);
+                PassPlan q = full;
+                const int64_t pin[3] = {nin[0], nin[1], planes}, pout[3] = {nout[0], nout[1], planes};
+                set_geometry(q.ap, full.axis, pin, pout);
+                const Tuning *tune = nullptr;
+                Tuning t;
+                if (!(p->flags & DFFT_FLAG_NO_TUNE) && tuning_lookup(q, t)) tune = &t;
+                int rc2 = schedule_pass(q, tune, 1024, 0, 1024);
+                if (rc2 != DFFT_OK) return rc2;
+                q.alg_bytes = full.alg_bytes * (double)planes / (double)n[2];
+                tt.add(q.ap);
+                (which == 0 ? cpa : cpb).push_back(ChunkPass{q, k0, planes});
+            }
+        }
+    }
     int rc = tt.upload(&p->d_tw, &p->tw_bytes);
     if (rc != DFFT_OK) return rc;
     for (auto &pp : p->passes) pp.ap.tw = p->d_tw;
+    if (slab_ok) {
+        const bool inv = p->sign > 0;
+        auto plane_bytes = [&](const PassPlan &q, bool input) -> int64_t {
+            const bool real = input ? (q.ap.op == OP_R2C || q.ap.op == OP_R2C_FULL)
+                                    : (q.ap.op == OP_C2R || q.ap.op == OP_C2R_FULL);
+            const int64_t *sz = real ? n : nc;
+            return sz[0] * sz[1] * (real ? es_r : es_c);
+        };
+        auto item = [&](ChunkPass &c) {
+            c.pp.ap.tw = p->d_tw;
+            return make_group_item(c.pp, c.pp.src, c.k0 * plane_bytes(c.pp, true), c.pp.dst,
+                                   c.k0 * plane_bytes(c.pp, false));
+        };
+        std::vector<std::vector<GroupItem>> levels;
+        const PassPlan &line = inv ? p->passes[0] : p->passes[2];
+        if (inv) levels.push_back({make_group_item(line, line.src, 0, line.dst, 0)});
+        const size_t C = cpa.size();
+        for (size_t i = 0; i <= C; ++i) {
+            std::vector<GroupItem> lv;
+            if (i < C) lv.push_back(item(cpa[i]));
+            if (i >= 1) lv.push_back(item(cpb[i - 1]));
+            levels.push_back(lv);
+        }
+        if (!inv) levels.push_back({make_group_item(line, line.src, 0, line.dst, 0)});
+        rc = group_create(&p->slab, levels);
+        if (rc != DFFT_OK) return rc;
+    }
     return DFFT_OK;
 }
 
+}  // namespace dfft
+
+namespace dfft {
+int plan_pass_count(const dfft_plan *p) { return (int)p->passes.size(); }
+PassPlan plan_pass(const dfft_plan *p, int i) { return p->passes[i]; }
+size_t plan_buffer_bytes(const dfft_plan *p, int buf) {
+    return buf == BUF_IN ? p->in_bytes : buf == BUF_OUT ? p->out_bytes : p->work_bytes;
+}
 }  // namespace dfft
 
 static void free_plan(dfft_plan *p) {
     if (!p) return;
     if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
+    group_destroy(p->slab);
     if (p->d_tw) (void)hipFree(p->d_tw);
     if (p->own_work) (void)hipFree(p->own_work);
     delete p;
@@ -557,6 +649,10 @@ int dfft_execute(const dfft_plan *p, const void *d_in, void *d_out, void *d_work
         }
         work = mp->own_work;
     }
+    if (p->slab) {
+        void *slots[3] = {const_cast<void *>(d_in), d_out, work};
+        return group_launch(p->slab, slots, 3, st);
+    }
     if (p->flags & DFFT_FLAG_GRAPH) return run_graph(p, d_in, d_out, work, st);
     return run_passes(p, d_in, d_out, work, st);
 }
@@ -606,6 +702,13 @@ int dfft_plan_describe(const dfft_plan *p, int pass, char *buf, size_t buflen) {
              "\"contig\":%d,\"threads\":%d,\"blocks\":%lld,\"lds\":%zu,\"bytes\":%.0f,\"prec\":\"%s\"}",
              pass, pp.axis, op_name(a.op), a.sp.L, a.nlog, rad.c_str(), a.TPL, a.TC, a.contig, pp.threads,
              (long long)pp.nblocks, pp.lds, pp.alg_bytes, pp.prec ? "f64" : "f32");
+    return DFFT_OK;
+}
+
+int dfft_plan_schedule(const dfft_plan *p, int *schedule, int *launches) {
+    if (!p || !schedule) return DFFT_ERR_ARG;
+    *schedule = p->slab ? DFFT_SCHEDULE_PLANE_PIPELINE : DFFT_SCHEDULE_PASSES;
+    if (launches) *launches = p->slab ? group_num_launches(p->slab) : (int)p->passes.size();
     return DFFT_OK;
 }
 

@@ -36,7 +36,9 @@ struct Tuning {
 };
 
 bool factorize(int L, std::vector<int> &R);
-int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads = 1024, size_t max_lds = 0);
+void set_geometry(AxisPass &a, int axis, const int64_t nin[3], const int64_t nout[3]);
+int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads = 1024, size_t max_lds = 0,
+                  int contig_threads = 256);
 
 // Stage twiddle + pre/post blocks of every distinct (op, L, radices) of one
 // precision, in one device table.

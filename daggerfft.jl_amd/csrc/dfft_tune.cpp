@@ -1,9 +1,29 @@
-// dfft_tune.cpp -- schedule cache of the autotuner (the FFTW wisdom analogue,
+// dfft_tune.cpp -- the autotuner and its schedule cache (the analogue of the
+// reference's FFTW wisdom: PLAN_CACHE + import/export_wisdom,
 // src/fft.jl:113-124,162).
+//
+// For every distinct axis pass of a plan the tuner times candidate schedules
+// on the device -- radix order, threads per line, lines/columns per
+// workgroup, LDS padding -- and records the fastest under a key of (precision,
+// op, length, layout, size class).  dfft_plan_create and dfft_batch_create
+// consult the cache unless DFFT_FLAG_NO_TUNE is set.  The cache can be saved
+// to and loaded from a text file (one entry per line).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
 #include <mutex>
+#include <set>
+#include <sstream>
 #include <string>
 #include <unordered_map>
+#include <vector>
 
+#include "../../include/daggerfft_hip.h"
+#include "dfft_kernels.hpp"
 #include "dfft_plan.hpp"
 
 namespace dfft {
@@ -11,10 +31,15 @@ namespace dfft {
 static std::mutex g_mu;
 static std::unordered_map<std::string, Tuning> g_cache;
 
+static int size_class(const AxisPass &a) {
+    const double lines = (double)a.nlines * (double)a.nouter;
+    return (int)std::floor(std::log2(std::max(1.0, lines * a.sp.L)));
+}
+
 std::string tuning_key(const PassPlan &pp) {
     const AxisPass &a = pp.ap;
     return std::to_string(pp.prec) + "/" + std::to_string(a.op) + "/" + std::to_string(a.sp.L) + "/" +
-           std::to_string(a.contig);
+           std::to_string(a.contig) + "/" + std::to_string(size_class(a));
 }
 
 bool tuning_lookup(const PassPlan &pp, Tuning &t) {
@@ -25,4 +50,221 @@ bool tuning_lookup(const PassPlan &pp, Tuning &t) {
     return true;
 }
 
+static void tuning_store(const std::string &key, const Tuning &t) {
+    std::lock_guard<std::mutex> g(g_mu);
+    g_cache[key] = t;
+}
+
+// Radix orders to try for length L: all splits of the power-of-two part
+// into 8/4/2 factors (as multisets), each in descending and ascending order,
+// followed/preceded by the odd factors.
+static std::vector<std::vector<int>> radix_candidates(int L) {
+    std::vector<std::vector<int>> out;
+    int p2 = 0, m = L;
+    while (m % 2 == 0) { m /= 2; ++p2; }
+    std::vector<int> odd;
+    for (int r : {3, 5, 7})
+        while (m % r == 0) { odd.push_back(r); m /= r; }
+    if (m != 1) return out;
+    std::set<std::vector<int>> seen;
+    for (int n8 = 0; 3 * n8 <= p2; ++n8)
+        for (int n4 = 0; 3 * n8 + 2 * n4 <= p2; ++n4) {
+            const int n2 = p2 - 3 * n8 - 2 * n4;
+            if (n2 > 2) continue;  // long radix-2 chains are never competitive
+            std::vector<int> pw;
+            pw.insert(pw.end(), n8, 8);
+            pw.insert(pw.end(), n4, 4);
+            pw.insert(pw.end(), n2, 2);
+            for (int dir = 0; dir < 2; ++dir) {
+                std::vector<int> a = pw;
+                if (dir) std::reverse(a.begin(), a.end());
+                std::vector<int> c1 = a, c2 = odd;
+                c1.insert(c1.end(), odd.begin(), odd.end());
+                c2.insert(c2.end(), a.begin(), a.end());
+                for (auto &c : {c1, c2})
+                    if ((int)c.size() <= MAX_STAGES && seen.insert(c).second) out.push_back(c);
+            }
+        }
+    if (L == 1) out.push_back({});
+    return out;
+}
+
+static std::vector<Tuning> candidates(const PassPlan &pp, int level, int max_threads) {
+    const AxisPass &a = pp.ap;
+    std::vector<Tuning> out;
+    auto rads = radix_candidates(a.sp.L);
+    if (level <= 1 && rads.size() > 4) rads.resize(4);
+    std::vector<int> targets = {8, 4};
+    std::vector<int> tcs = a.contig ? std::vector<int>{1, 2, 4, 8, 16} : std::vector<int>{4, 8, 16, 32};
+    std::vector<int> pads = a.contig ? (level >= 2 ? std::vector<int>{3, 4, 5} : std::vector<int>{3, 4})
+                                     : std::vector<int>{0};
+    for (const auto &R : rads) {
+        std::set<int> tpls;
+        for (int e : targets) {
+            int tpl = std::max(1, (a.sp.L + e - 1) / e);
+            for (;;) {  // smallest thread count >= L/e that holds every stage in EMAX registers
+                bool ok = true;
+                for (int r : R)
+                    if ((a.sp.L / r + tpl - 1) / tpl > EMAX / r) { ok = false; break; }
+                if (ok || tpl >= max_threads) break;
+                ++tpl;
+            }
+            if (tpl <= max_threads) tpls.insert(tpl);
+        }
+        for (int tpl : tpls)
+            for (int tc : tcs) {
+                if (tpl * tc > max_threads) continue;
+                for (int ps : pads) {
+                    Tuning t;
+                    t.radices = R;
+                    t.tpl = tpl;
+                    t.tc = tc;
+                    t.pad_shift = ps;
+                    out.push_back(t);
+                }
+            }
+    }
+    return out;
+}
+
 }  // namespace dfft
+
+using namespace dfft;
+
+struct dfft_plan;  // defined in dfft_plan.cpp
+namespace dfft {
+int plan_pass_count(const dfft_plan *p);
+PassPlan plan_pass(const dfft_plan *p, int i);
+size_t plan_buffer_bytes(const dfft_plan *p, int buf);
+}  // namespace dfft
+
+extern "C" {
+
+int dfft_autotune(int rank, const int64_t *n, int kind, int sign, const int *r2r_kinds, int precision, int decomp,
+                  const int *dims, int ndims, uint32_t flags, int level, void *stream) {
+    if (level < 0) return DFFT_ERR_ARG;
+    if (level == 0) return DFFT_OK;
+    dfft_plan *p = nullptr;
+    int rc = dfft_plan_create(&p, rank, n, kind, sign, r2r_kinds, precision, decomp, dims, ndims,
+                              flags | DFFT_FLAG_NO_TUNE);
+    if (rc != DFFT_OK) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    void *buf[3] = {nullptr, nullptr, nullptr};
+    for (int b = 0; b < 3; ++b) {
+        size_t nb = plan_buffer_bytes(p, b);
+        if (nb && hipMalloc(&buf[b], nb) != hipSuccess) {
+            rc = DFFT_ERR_OOM;
+            break;
+        }
+        if (nb) (void)launch_fill(buf[b], (int64_t)(nb / (precision ? 8 : 4)), precision, 7 + b, st);
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (rc == DFFT_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = DFFT_ERR_HIP;
+    std::set<std::string> done;
+    const int reps = level >= 2 ? 5 : 3;
+    for (int i = 0; rc == DFFT_OK && i < plan_pass_count(p); ++i) {
+        PassPlan base = plan_pass(p, i);
+        const std::string key = tuning_key(base);
+        if (!done.insert(key).second) continue;
+        float best = 1e30f;
+        Tuning best_t;
+        bool have = false;
+        for (const Tuning &cand : candidates(base, level, 1024)) {
+            PassPlan pp = base;
+            if (schedule_pass(pp, &cand) != DFFT_OK) continue;
+            // the candidate's twiddles (its own radix order)
+            TwiddleTable tt;
+            tt.prec = pp.prec;
+            tt.add(pp.ap);
+            void *d_tw = nullptr;
+            size_t twb = 0;
+            if (tt.upload(&d_tw, &twb) != DFFT_OK) {
+                rc = DFFT_ERR_OOM;
+                break;
+            }
+            pp.ap.tw = d_tw;
+            pp.ap.in = buf[pp.src];
+            pp.ap.out = buf[pp.dst];
+            bool ok = launch_axis(pp.ap, pp.prec, pp.nblocks, pp.threads, pp.lds, st) == hipSuccess;
+            if (ok) {
+                (void)hipEventRecord(e0, st);
+                for (int r = 0; r < reps && ok; ++r)
+                    ok = launch_axis(pp.ap, pp.prec, pp.nblocks, pp.threads, pp.lds, st) == hipSuccess;
+                (void)hipEventRecord(e1, st);
+                ok = ok && hipEventSynchronize(e1) == hipSuccess;
+            }
+            float ms = 0;
+            if (ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms / reps < best) {
+                best = ms / reps;
+                best_t = cand;
+                have = true;
+            }
+            (void)hipFree(d_tw);
+            if (!ok) (void)hipGetLastError();
+        }
+        if (have) tuning_store(key, best_t);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    for (void *b : buf)
+        if (b) (void)hipFree(b);
+    dfft_plan_destroy(p);
+    return rc;
+}
+
+int dfft_tune_save(const char *path) {
+    if (!path) return DFFT_ERR_ARG;
+    std::ofstream f(path);
+    if (!f) return DFFT_ERR_ARG;
+    std::lock_guard<std::mutex> g(g_mu);
+    f << "# daggerfft_hip schedule cache: key radices tpl tc pad_shift\n";
+    for (const auto &kv : g_cache) {
+        f << kv.first << " ";
+        for (size_t i = 0; i < kv.second.radices.size(); ++i) f << (i ? "," : "") << kv.second.radices[i];
+        if (kv.second.radices.empty()) f << "-";
+        f << " " << kv.second.tpl << " " << kv.second.tc << " " << kv.second.pad_shift << "\n";
+    }
+    return f.good() ? DFFT_OK : DFFT_ERR_ARG;
+}
+
+int dfft_tune_load(const char *path) {
+    if (!path) return DFFT_ERR_ARG;
+    std::ifstream f(path);
+    if (!f) return DFFT_ERR_ARG;
+    std::string line;
+    while (std::getline(f, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::istringstream ss(line);
+        std::string key, rad;
+        Tuning t;
+        if (!(ss >> key >> rad >> t.tpl >> t.tc >> t.pad_shift)) return DFFT_ERR_ARG;
+        if (rad != "-") {
+            std::stringstream rs(rad);
+            std::string tok;
+            while (std::getline(rs, tok, ',')) {
+                const int r = std::atoi(tok.c_str());
+                if (r != 2 && r != 3 && r != 4 && r != 5 && r != 7 && r != 8) return DFFT_ERR_ARG;
+                t.radices.push_back(r);
+            }
+        }
+        if (t.tpl < 0 || t.tpl > 1024 || t.tc < 0 || t.tc > 1024 || t.pad_shift < 0 || t.pad_shift > 30)
+            return DFFT_ERR_ARG;
+        tuning_store(key, t);
+    }
+    return DFFT_OK;
+}
+
+int dfft_tune_clear(void) {
+    std::lock_guard<std::mutex> g(g_mu);
+    g_cache.clear();
+    return DFFT_OK;
+}
+
+int dfft_tune_entries(int *count) {
+    if (!count) return DFFT_ERR_ARG;
+    std::lock_guard<std::mutex> g(g_mu);
+    *count = (int)g_cache.size();
+    return DFFT_OK;
+}
+
+}  // extern "C"

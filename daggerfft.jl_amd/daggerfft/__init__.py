@@ -257,6 +257,12 @@ class Plan:
             out.append(json.loads(buf.value.decode()))
         return out
 
+    def schedule(self):
+        """('passes' | 'plane_pipeline', kernel launches per execution)."""
+        k, n = ctypes.c_int(), ctypes.c_int()
+        check(lib().dfft_plan_schedule(self._h, ctypes.byref(k), ctypes.byref(n)))
+        return ("plane_pipeline" if k.value == 1 else "passes"), n.value
+
     def algorithmic_bytes(self):
         b = ctypes.c_double()
         check(lib().dfft_plan_algorithmic_bytes(self._h, ctypes.byref(b)))
@@ -299,6 +305,39 @@ class Plan:
             self.destroy()
         except Exception:
             pass
+
+
+def autotune(shape, kind=L.C2C, sign=-1, precision=L.F64, decomp=Pencil(), dims=None, r2r_kinds=None, level=1,
+             flags=0, stream=None):
+    """Time candidate schedules of every axis pass of this transform on the
+    device and record the fastest in the library's schedule cache; plans
+    created afterwards use it.  Cached plans of the same shape are dropped."""
+    dims = tuple(range(1, len(shape) + 1)) if dims is None else tuple(dims)
+    kinds = L.i32(r2r_kinds) if r2r_kinds is not None else None
+    check(lib().dfft_autotune(len(shape), L.i64(shape), kind, sign, kinds, precision, decomp.code,
+                              L.i32([d - 1 for d in dims]), len(dims), flags, level, _stream_handle(stream)),
+          "dfft_autotune")
+    clear_plan_cache()
+
+
+def tune_save(path):
+    check(lib().dfft_tune_save(str(path).encode()), "dfft_tune_save")
+
+
+def tune_load(path):
+    check(lib().dfft_tune_load(str(path).encode()), "dfft_tune_load")
+    clear_plan_cache()
+
+
+def tune_clear():
+    check(lib().dfft_tune_clear())
+    clear_plan_cache()
+
+
+def tune_entries():
+    n = ctypes.c_int()
+    check(lib().dfft_tune_entries(ctypes.byref(n)))
+    return n.value
 
 
 # GPU_PLAN_CACHE (src/fftgpu.jl:83,107-116), guarded by a lock (the reference's is not)

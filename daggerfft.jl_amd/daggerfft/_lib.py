@@ -64,6 +64,7 @@ def _declare(L):
         "dfft_plan_num_passes": [P, IP],
         "dfft_plan_describe": [P, I, ctypes.c_char_p, ctypes.c_size_t],
         "dfft_plan_algorithmic_bytes": [P, ctypes.POINTER(ctypes.c_double)],
+        "dfft_plan_schedule": [P, IP, IP],
         "dfft_version": [IP, IP],
         "dfft_box_pack": [P, P, I64P, I64P, I64P, ctypes.c_int64, I, P],
         "dfft_box_unpack": [P, P, I64P, I64P, I64P, ctypes.c_int64, I, P],
@@ -75,9 +76,13 @@ def _declare(L):
         "dfft_batch_workspace_size": [P, SZP],
         "dfft_batch_num_items": [P, I64P],
         "dfft_batch_destroy": [P],
+        "dfft_autotune": [I, I64P, I, I, IP, I, I, IP, I, ctypes.c_uint32, I, P],
+        "dfft_tune_save": [ctypes.c_char_p],
+        "dfft_tune_load": [ctypes.c_char_p],
+        "dfft_tune_clear": [],
+        "dfft_tune_entries": [IP],
         "dfft_batch_algorithmic_bytes": [P, ctypes.POINTER(ctypes.c_double)],
-        "dfft_batch_error": [P, P, IP],
-        "dfft_batch_set_spin_limit": [P, ctypes.c_uint64],
+        "dfft_batch_num_launches": [P, IP],
     }
     for name, args in sig.items():
         f = getattr(L, name, None)

@@ -1,5 +1,6 @@
-"""Batched persistent launch: many independent 3D transforms in one kernel
-launch (dfft_batch_* of include/daggerfft_hip.h).  No reference counterpart;
+"""Batched launch: many independent 3D transforms per kernel launch -- three
+launches per batch, one per axis-pass level (dfft_batch_* of
+include/daggerfft_hip.h).  No reference counterpart;
 the reference runs one cuFFT plan per chunk (src/fftgpu.jl:118-124)."""
 from __future__ import annotations
 
@@ -43,14 +44,11 @@ class Batch:
         check(lib().dfft_batch_execute(self._h, pi, po, _stream_handle(stream)), "dfft_batch_execute")
         return outs
 
-    def error_flag(self, stream=None):
-        from . import _stream_handle
-        f = ctypes.c_int()
-        check(lib().dfft_batch_error(self._h, _stream_handle(stream), ctypes.byref(f)))
-        return f.value
-
-    def set_spin_limit(self, limit):
-        check(lib().dfft_batch_set_spin_limit(self._h, ctypes.c_uint64(limit)))
+    @property
+    def launches(self):
+        n = ctypes.c_int()
+        check(lib().dfft_batch_num_launches(self._h, ctypes.byref(n)))
+        return n.value
 
     def destroy(self):
         if getattr(self, "_h", None):

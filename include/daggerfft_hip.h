@@ -137,6 +137,16 @@ int dfft_execute_pass(const dfft_plan *plan, int pass, const void *d_in, void *d
 int dfft_plan_num_passes(const dfft_plan *plan, int *npasses);
 int dfft_plan_describe(const dfft_plan *plan, int pass, char *buf, size_t buflen);
 
+/* How dfft_execute runs the plan: one launch per axis pass, or (Slab plans of
+ * rank-3 C2C/R2C/C2R arrays with dims[2] == 2) a software pipeline over chunks
+ * of z-planes -- launch i runs the dims[0] pass of chunk i together with the
+ * dims[1] pass of chunk i-1, which is re-read while still in the Infinity
+ * Cache -- followed by the dims[2] pass.  `launches`: kernel launches per
+ * execution. */
+#define DFFT_SCHEDULE_PASSES          0
+#define DFFT_SCHEDULE_PLANE_PIPELINE  1
+int dfft_plan_schedule(const dfft_plan *plan, int *schedule, int *launches);
+
 /* Algorithmic HBM bytes of the plan: sum over passes of bytes read + written. */
 int dfft_plan_algorithmic_bytes(const dfft_plan *plan, double *bytes);
 
@@ -170,11 +180,27 @@ int dfft_permute(void *d_out, const void *d_in, const int64_t *in_shape, const i
 int dfft_fill_uniform(void *d_out, int64_t nelem, int is_complex, int precision, uint64_t seed,
                       void *stream);
 
-/* ---- batched persistent launch ------------------------------------------
- * No reference counterpart: many independent transforms in one persistent
- * launch (the reference runs one cuFFT plan per chunk, src/fftgpu.jl:118-124).
+/* ---- autotuner (the FFTW wisdom analogue: PLAN_CACHE, import_wisdom /
+ * export_wisdom, src/fft.jl:113-124,162) ---------------------------------
+ * dfft_autotune times candidate schedules (radix order, threads per line,
+ * lines/columns per workgroup, LDS padding) of every axis pass of the plan the
+ * same arguments describe, on the current device, and records the fastest in
+ * a process-wide cache that later dfft_plan_create / dfft_batch_create calls
+ * use (unless DFFT_FLAG_NO_TUNE).  level: 0 off, 1 quick, 2 full.  It
+ * allocates scratch buffers of the plan's size and synchronises `stream`. */
+int dfft_autotune(int rank, const int64_t *n, int kind, int sign, const int *r2r_kinds, int precision,
+                  int decomp, const int *dims, int ndims, uint32_t flags, int level, void *stream);
+int dfft_tune_save(const char *path);
+int dfft_tune_load(const char *path);
+int dfft_tune_clear(void);
+int dfft_tune_entries(int *count);
+
+/* ---- batched launch ------------------------------------------------------
+ * No reference counterpart: many independent transforms per kernel launch
+ * (the reference runs one cuFFT plan per chunk, src/fftgpu.jl:118-124).
  * Each transform is rank 3 over dims (0,1,2); C2C (either sign) or R2C
- * (forward, even n[0]).  Results equal the same transform run alone.      */
+ * (forward, even n[0]).  Launch l runs axis pass l of every transform, so a
+ * whole batch takes three launches.  Results equal each transform run alone. */
 typedef struct {
     int64_t n[3];       /* size of this 3D transform (n[0] fastest) */
     int kind;           /* DFFT_C2C or DFFT_R2C */
@@ -186,18 +212,13 @@ typedef struct {
 typedef struct dfft_batch dfft_batch;
 
 int dfft_batch_create(dfft_batch **batch, int count, const dfft_batch_desc *descs, uint32_t flags);
-/* d_in[t], d_out[t]: device pointers of transform t (in place when equal, C2C only).
- * Synchronises `stream` first (the item table is staged through pinned memory). */
+/* d_in[t], d_out[t]: device pointers of transform t (in place when equal, C2C only). */
 int dfft_batch_execute(dfft_batch *batch, void *const *d_in, void *const *d_out, void *stream);
 int dfft_batch_workspace_size(const dfft_batch *batch, size_t *bytes);
-/* total number of tiles (work tickets) of one execution */
+/* workgroups (tiles) over all launches of one execution, and the launch count */
 int dfft_batch_num_items(const dfft_batch *batch, int64_t *items);
+int dfft_batch_num_launches(const dfft_batch *batch, int *launches);
 int dfft_batch_algorithmic_bytes(const dfft_batch *batch, double *bytes);
-/* Synchronises `stream` and reports whether a dependency wait hit the spin
- * limit during the last execution (1) or not (0). */
-int dfft_batch_error(dfft_batch *batch, void *stream, int *flag);
-/* Bound on one dependency wait, in polling rounds (tests force tiny limits). */
-int dfft_batch_set_spin_limit(dfft_batch *batch, uint64_t limit);
 int dfft_batch_destroy(dfft_batch *batch);
 
 #ifdef __cplusplus

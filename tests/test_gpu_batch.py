@@ -1,7 +1,13 @@
-"""Batched persistent launch (64 mixed-size 3D transforms in one launch):
-every result equals the same transform planned and run alone, and the
-oracle within tolerance (ComplexF64 rel-L2 <= 1e-12; ComplexF32 vs the FP64
-oracle <= 1e-5)."""
+"""Batched launch of 64 mixed-size 3D transforms: every result equals the
+same transform planned and run alone, and the oracle within tolerance
+(ComplexF64 rel-L2 <= 1e-12; ComplexF32 vs the FP64 oracle <= 1e-5).
+
+The first three tests exercised an earlier single persistent launch whose
+workgroups waited on each other through device counters.  That kernel hung
+on the GPU twice (the second time on the run meant to test a fix); no cause
+was found, so the kernel was removed and these tests stay skipped.  The
+batch now runs as grouped launches (one per axis-pass level, no waits
+between workgroups), tested by the test_grouped_* functions below."""
 import numpy as np
 import pytest
 
@@ -30,6 +36,7 @@ def _run_batch(descs, seeds):
     return b, ins, outs
 
 
+@pytest.mark.skip(reason="persistent inter-workgroup-wait batch kernel hung on the GPU twice; removed")
 def test_batch_config_64_mixed(gpu):
     import daggerfft as D
     from daggerfft import _lib as L
@@ -55,6 +62,7 @@ def test_batch_config_64_mixed(gpu):
     assert worst64 <= 1e-12 and worst32 <= 1e-5
 
 
+@pytest.mark.skip(reason="persistent inter-workgroup-wait batch kernel hung on the GPU twice; removed")
 def test_batch_inverse_and_r2c_mix(gpu):
     import daggerfft as D
     from daggerfft import _lib as L
@@ -71,6 +79,7 @@ def test_batch_inverse_and_r2c_mix(gpu):
         assert O.rel_l2(Y.to_host(), alone.to_host()) <= tol
 
 
+@pytest.mark.skip(reason="persistent inter-workgroup-wait batch kernel hung on the GPU twice; removed")
 def test_batch_spin_limit_reports_error(gpu):
     """A tiny spin limit forced from the host makes dependency waits give up:
     the kernel still drains and the error flag is raised."""
@@ -88,3 +97,61 @@ def test_batch_spin_limit_reports_error(gpu):
     b.execute(ins, outs)
     assert b.error_flag() == 0
     assert O.rel_l2(outs[0].to_host(), D.fft(ins[0]).to_host()) < 1e-15
+
+
+def _run_grouped(descs, seeds):
+    import daggerfft as D
+    from daggerfft.batch import Batch
+    b = Batch(descs)
+    assert b.launches == 3
+    ins, outs = [], []
+    for (shape, kind, sign, prec), sd in zip(descs, seeds):
+        if kind == 0:
+            dt = np.complex128 if prec == 1 else np.complex64
+            ins.append(D.rand(shape, dt, seed=sd))
+            outs.append(D.DeviceArray(shape, dt))
+        else:
+            dt = np.float64 if prec == 1 else np.float32
+            ins.append(D.rand(shape, dt, seed=sd))
+            oshape = (shape[0] // 2 + 1,) + tuple(shape[1:])
+            outs.append(D.DeviceArray(oshape, np.complex128 if prec == 1 else np.complex64))
+    b.execute(ins, outs)
+    return b, ins, outs
+
+
+def test_grouped_batch_small_mix(gpu):
+    import daggerfft as D
+    from daggerfft import _lib as L
+    descs = [((32, 48, 20), L.C2C, +1, 1), ((64, 30, 14), L.R2C, -1, 1), ((16, 16, 16), L.C2C, -1, 0),
+             ((40, 21, 10), L.R2C, -1, 0), ((7, 5, 3), L.C2C, -1, 1)]
+    b, ins, outs = _run_grouped(descs, [51, 52, 53, 54, 55])
+    for (shape, kind, sign, prec), X, Y in zip(descs, ins, outs):
+        if kind == L.C2C:
+            alone = D.fft(X) if sign < 0 else D.ifft(X)
+        else:
+            alone = D.rfft(X)
+        tol = 1e-15 if prec == 1 else 1e-7
+        assert O.rel_l2(Y.to_host(), alone.to_host()) <= tol
+
+
+def test_grouped_batch_config_64_mixed(gpu):
+    import daggerfft as D
+    from daggerfft import _lib as L
+    from daggerfft.batch import batch_config_sizes
+    cfg = batch_config_sizes(64, 7)
+    descs = [(shape, L.C2C, -1, prec) for shape, prec in cfg]
+    b, ins, outs = _run_grouped(descs, [1000 + t for t in range(64)])
+    worst64 = worst32 = 0.0
+    for t, ((shape, prec), X, Y) in enumerate(zip(cfg, ins, outs)):
+        yb = Y.to_host()
+        assert O.rel_l2(yb, D.fft(X).to_host()) < 1e-15, t
+        if t % 8 == 0 or t < 4:
+            dt = np.complex128 if prec == 1 else np.complex64
+            a = O.fill(1000 + t, shape, dt).astype(np.complex128)
+            e = O.rel_l2(yb.astype(np.complex128), O.fftn(a))
+            if prec == 1:
+                worst64 = max(worst64, e)
+            else:
+                worst32 = max(worst32, e)
+    print("grouped batch of 64: worst rel-L2 f64 %.3e, f32 %.3e, workgroups %d" % (worst64, worst32, b.tickets))
+    assert worst64 <= 1e-12 and worst32 <= 1e-5
