@@ -42,6 +42,21 @@ def parse():
     return ap.parse_args()
 
 
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE runs, calibrated on kernels of known size)."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_512cubed_c2c_f64.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch "):
+                name, val = line[len("traffic_bytes_per_launch "):].rsplit("=", 1)
+                if name.strip() == kernel.replace(" ", ""):
+                    return float(val)
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(n):
     """The oracle's restatement of the reference pipeline (per-axis batched
     1D FFTs in stage order) on the host cores, one full n^3 forward + inverse."""
@@ -130,6 +145,7 @@ def main():
             gbps = desc["bytes"] / (avg / 1e3) / 1e9 if avg else None
             passes.append({"dir": "fwd" if k[0] == "f" else "inv", "axis": desc["axis"] + 1, "op": desc["op"],
                            "L": desc["L"], "radices": desc["radices"], "TPL": desc["TPL"], "TC": desc["TC"],
+                           "variant": desc["variant"], "kernel": desc["kernel"],
                            "bytes": desc["bytes"], "avg_ms": round(avg, 4) if avg else None,
                            "GBps": round(gbps, 1) if gbps else None,
                            "frac": round(gbps / HBM_PEAK_GBPS, 4) if gbps else None})
@@ -181,21 +197,14 @@ def main():
     t_transform_s = ms_step / 2 / 1e3
     gflops = 5 * N * math.log2(N) * 1e-9 / t_transform_s
 
+    # Dominant kernel: the slowest axis pass of the per-pass schedule (Pencil and
+    # Slab run the same axis-pass kernels; Slab only groups them differently).
     roofline = None
-    if best == "slab":
-        avg = (r["fwd_ms"] + r["inv_ms"]) / 2
-        ach = r["bytes"] / (avg / 1e3) / 1e9
-        roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                    "frac": round(ach / HBM_PEAK_GBPS, 4), "traffic": None,
-                    "kernel": "k_group<1024> (Slab: 2D stage software-pipelined over z-plane chunks, then "
-                              "the axis-3 pass; all launches of one transform)",
-                    "bytes_per_launch": r["bytes"], "avg_ms": round(avg, 4)}
-    elif not args.profile_steps:
-        dom = max(r["passes"], key=lambda p: p["avg_ms"])
+    if "pencil" in results and not args.profile_steps:
+        dom = max(results["pencil"]["passes"], key=lambda p: p["avg_ms"])
         roofline = {"bound": "hbm", "achieved": dom["GBps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                    "frac": dom["frac"], "traffic": None,
-                    "kernel": "k_axis<double,C2C,%s> axis %d (%s)" % (
-                        "contig" if dom["axis"] == 1 else "strided", dom["axis"], dom["dir"]),
+                    "frac": dom["frac"], "traffic": pmc_traffic(dom["kernel"]),
+                    "kernel": "%s (axis %d, %s)" % (dom["kernel"], dom["axis"], dom["dir"]),
                     "bytes_per_launch": dom["bytes"], "avg_ms": dom["avg_ms"]}
 
     cfg = {"workload": "3D C2C ComplexF64 %dx%dx%d forward+inverse" % (n, n, n), "decomp": best,

@@ -21,10 +21,41 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "dfft_math.hpp"
 #include "dfft_types.hpp"
 
 namespace dfft {
+
+// Global element access.  Streaming (nontemporal) accesses skip cache
+// allocation: each element of a pass is read once and written once.
+template <class T> struct vec2;
+template <> struct vec2<double> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct vec2<float> { typedef float type __attribute__((ext_vector_type(2))); };
+
+template <class T>
+__device__ __forceinline__ cx<T> gload(const cx<T> *p, bool nt) {
+    typedef typename vec2<T>::type V;
+    V v = nt ? __builtin_nontemporal_load(reinterpret_cast<const V *>(p)) : *reinterpret_cast<const V *>(p);
+    return cx<T>{v.x, v.y};
+}
+template <class T>
+__device__ __forceinline__ void gstore(cx<T> *p, cx<T> x, bool nt) {
+    typedef typename vec2<T>::type V;
+    V v = {x.x, x.y};
+    if (nt) __builtin_nontemporal_store(v, reinterpret_cast<V *>(p));
+    else *reinterpret_cast<V *>(p) = v;
+}
+template <class T>
+__device__ __forceinline__ T gload(const T *p, bool nt) {
+    return nt ? __builtin_nontemporal_load(p) : *p;
+}
+template <class T>
+__device__ __forceinline__ void gstore(T *p, T x, bool nt) {
+    if (nt) __builtin_nontemporal_store(x, p);
+    else *p = x;
+}
 
 template <class T, int OP, bool CONTIG>
 struct TileCtx {
@@ -51,19 +82,20 @@ struct TileCtx {
         const cx<T> *inc = reinterpret_cast<const cx<T> *>(in);
         if constexpr (OP == OP_C2C) {
             if (!valid) return cx<T>{0, 0};
-            cx<T> v = inc[ib + (int64_t)p * a.in_sm];
+            cx<T> v = gload(inc + ib + (int64_t)p * a.in_sm, a.nt);
             return a.inverse ? conj(v) : v;
         } else if constexpr (OP == OP_R2C) {
             if (!valid) return cx<T>{0, 0};
             if constexpr (CONTIG) {
                 // adjacent reals (2p, 2p+1) of a unit-stride line: one vector load
-                return inc[(ib >> 1) + p];
+                return gload(inc + (ib >> 1) + p, a.nt);
             } else {
-                return cx<T>{inr[ib + (int64_t)(2 * p) * a.in_sm], inr[ib + (int64_t)(2 * p + 1) * a.in_sm]};
+                return cx<T>{gload(inr + ib + (int64_t)(2 * p) * a.in_sm, a.nt),
+                             gload(inr + ib + (int64_t)(2 * p + 1) * a.in_sm, a.nt)};
             }
         } else if constexpr (OP == OP_R2C_FULL) {
             if (!valid) return cx<T>{0, 0};
-            return cx<T>{inr[ib + (int64_t)p * a.in_sm], (T)0};
+            return cx<T>{gload(inr + ib + (int64_t)p * a.in_sm, a.nt), (T)0};
         } else if constexpr (OP == OP_C2R) {
             // the H+1 input bins sit in LDS; Z_p = A + i w^-p B (conjugated: the
             // inverse H-point FFT runs as a forward FFT on conjugates)
@@ -78,20 +110,20 @@ struct TileCtx {
             if (!valid) return cx<T>{0, 0};
             const int L = a.sp.L;
             cx<T> x;
-            if (2 * p <= L) x = inc[ib + (int64_t)p * a.in_sm];
-            else x = conj(inc[ib + (int64_t)(L - p) * a.in_sm]);
+            if (2 * p <= L) x = gload(inc + ib + (int64_t)p * a.in_sm, a.nt);
+            else x = conj(gload(inc + ib + (int64_t)(L - p) * a.in_sm, a.nt));
             if (p == 0 || 2 * p == L) x.y = 0;
             return conj(x);
         } else if constexpr (OP == OP_DCT2) {
             if (!valid) return cx<T>{0, 0};
             const int L = a.sp.L;
             int q = (p < (L + 1) / 2) ? 2 * p : 2 * (L - 1 - p) + 1;
-            return cx<T>{inr[ib + (int64_t)q * a.in_sm], (T)0};
+            return cx<T>{gload(inr + ib + (int64_t)q * a.in_sm, a.nt), (T)0};
         } else {  // OP_DCT3
             if (!valid) return cx<T>{0, 0};
             const int L = a.sp.L;
-            T xp = inr[ib + (int64_t)p * a.in_sm];
-            T xr = p == 0 ? (T)0 : inr[ib + (int64_t)(L - p) * a.in_sm];
+            T xp = gload(inr + ib + (int64_t)p * a.in_sm, a.nt);
+            T xr = p == 0 ? (T)0 : gload(inr + ib + (int64_t)(L - p) * a.in_sm, a.nt);
             // conj( exp(+i pi p / 2L) (x_p - i x_{L-p}) ) = w_p (x_p + i x_{L-p})
             return cmul(post()[p], cx<T>{xp, xr});
         }
@@ -108,25 +140,25 @@ struct TileCtx {
             if (!valid) return;
             if constexpr (OP == OP_C2C) {
                 cx<T> o = a.inverse ? conj(v) : v;
-                outc[ob + (int64_t)p * a.out_sm] = scl(o, s);
+                gstore(outc + ob + (int64_t)p * a.out_sm, scl(o, s), a.nt);
             } else if constexpr (OP == OP_R2C_FULL) {
-                if (2 * p <= a.sp.L) outc[ob + (int64_t)p * a.out_sm] = scl(v, s);
+                if (2 * p <= a.sp.L) gstore(outc + ob + (int64_t)p * a.out_sm, scl(v, s), a.nt);
             } else if constexpr (OP == OP_C2R) {
                 if constexpr (CONTIG) {
-                    reinterpret_cast<cx<T> *>(out)[(ob >> 1) + p] = cx<T>{v.x * s, -v.y * s};
+                    gstore(reinterpret_cast<cx<T> *>(out) + (ob >> 1) + p, cx<T>{v.x * s, -v.y * s}, a.nt);
                 } else {
-                    outr[ob + (int64_t)(2 * p) * a.out_sm] = v.x * s;
-                    outr[ob + (int64_t)(2 * p + 1) * a.out_sm] = -v.y * s;
+                    gstore(outr + ob + (int64_t)(2 * p) * a.out_sm, v.x * s, a.nt);
+                    gstore(outr + ob + (int64_t)(2 * p + 1) * a.out_sm, -v.y * s, a.nt);
                 }
             } else if constexpr (OP == OP_C2R_FULL) {
-                outr[ob + (int64_t)p * a.out_sm] = v.x * s;
+                gstore(outr + ob + (int64_t)p * a.out_sm, v.x * s, a.nt);
             } else if constexpr (OP == OP_DCT2) {
                 cx<T> w = post()[p];
-                outr[ob + (int64_t)p * a.out_sm] = (T)2 * (w.x * v.x - w.y * v.y) * s;
+                gstore(outr + ob + (int64_t)p * a.out_sm, (T)2 * (w.x * v.x - w.y * v.y) * s, a.nt);
             } else {  // OP_DCT3
                 const int L = a.sp.L;
                 int q = (p < (L + 1) / 2) ? 2 * p : 2 * (L - 1 - p) + 1;
-                outr[ob + (int64_t)q * a.out_sm] = v.x * s;
+                gstore(outr + ob + (int64_t)q * a.out_sm, v.x * s, a.nt);
             }
         }
     }
@@ -196,13 +228,168 @@ __device__ __forceinline__ void dispatch_stage(const TileCtx<T, OP, CONTIG> &c, 
     }
 }
 
+// Fixed-schedule engine (compile-time length, radices and threads per line).
+// Per-thread registers: stage s with radix R keeps butterfly b (j = t + b*TPL,
+// j < L/R) in v[b*R .. b*R+R-1].  Between stages the values move through
+// LDS: either whole complex values (one round), or -- split mode, half the
+// LDS -- the real parts in one round and the imaginary parts in a second
+// round through the same buffer.
+enum LdsPart : int { PART_FULL = 0, PART_RE = 1, PART_IM = 2 };
+
+template <int R, int EMAX, int LC, int TPLC, class T, int OP, bool CONTIG>
+__device__ __forceinline__ void st_load_global(const TileCtx<T, OP, CONTIG> &c, cx<T> *v) {
+    constexpr int NBM = EMAX / R;
+    const int NB = LC / R, TPL = TPLC;
+#pragma unroll
+    for (int b = 0; b < NBM; ++b) {
+        const int j = c.t + b * TPL;
+        if (j < NB) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b * R + r] = c.load0(j + r * NB);
+        }
+    }
+}
+
+// twiddle by w_{Ns*R}^{r (j mod Ns)} (not in the first stage), then the R-point DFT
+template <int R, int EMAX, int LC, int TPLC, int Ns, class T, int OP, bool CONTIG>
+__device__ __forceinline__ void st_compute(const TileCtx<T, OP, CONTIG> &c, cx<T> *v, int twoff) {
+    constexpr bool twiddle = Ns > 1;
+    constexpr int NBM = EMAX / R;
+    const int NB = LC / R, TPL = TPLC;
+#pragma unroll
+    for (int b = 0; b < NBM; ++b) {
+        const int j = c.t + b * TPL;
+        if (j < NB) {
+            if constexpr (twiddle) {
+                const int k = j % Ns;
+                const cx<T> *w = reinterpret_cast<const cx<T> *>(c.a.tw) + twoff + k * (R - 1);
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[b * R + r] = cmul(v[b * R + r], w[r - 1]);
+            }
+            bfly<R>(v + b * R);
+        }
+    }
+}
+
+template <int R, int EMAX, int LC, int TPLC, class T, int OP, bool CONTIG>
+__device__ __forceinline__ void st_store_global(const TileCtx<T, OP, CONTIG> &c, const cx<T> *v) {
+    constexpr int NBM = EMAX / R;
+    const int NB = LC / R, TPL = TPLC;
+#pragma unroll
+    for (int b = 0; b < NBM; ++b) {
+        const int j = c.t + b * TPL;
+        if (j < NB) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) c.store_last(j + r * NB, v[b * R + r]);
+        }
+    }
+}
+
+// Stockham write of stage outputs: y[(j - j mod Ns) R + j mod Ns + r Ns]
+template <int R, int EMAX, int PART, int LC, int TPLC, int Ns, class T, int OP, bool CONTIG>
+__device__ __forceinline__ void st_write_lds(const TileCtx<T, OP, CONTIG> &c, const cx<T> *v) {
+    constexpr int NBM = EMAX / R;
+    const int NB = LC / R, TPL = TPLC;
+    T *ldsr = reinterpret_cast<T *>(c.lds);
+#pragma unroll
+    for (int b = 0; b < NBM; ++b) {
+        const int j = c.t + b * TPL;
+        if (j < NB) {
+            const int k = j % Ns;
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = c.li(base + r * Ns);
+                if constexpr (PART == PART_FULL) c.lds[i] = v[b * R + r];
+                else if constexpr (PART == PART_RE) ldsr[i] = v[b * R + r].x;
+                else ldsr[i] = v[b * R + r].y;
+            }
+        }
+    }
+}
+
+// Stockham read of stage inputs: x[j + r L/R]
+template <int R, int EMAX, int PART, int LC, int TPLC, class T, int OP, bool CONTIG>
+__device__ __forceinline__ void st_read_lds(const TileCtx<T, OP, CONTIG> &c, cx<T> *v) {
+    constexpr int NBM = EMAX / R;
+    const int NB = LC / R, TPL = TPLC;
+    const T *ldsr = reinterpret_cast<const T *>(c.lds);
+#pragma unroll
+    for (int b = 0; b < NBM; ++b) {
+        const int j = c.t + b * TPL;
+        if (j < NB) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = c.li(j + r * NB);
+                if constexpr (PART == PART_FULL) v[b * R + r] = c.lds[i];
+                else if constexpr (PART == PART_RE) v[b * R + r].x = ldsr[i];
+                else v[b * R + r].y = ldsr[i];
+            }
+        }
+    }
+}
+
+template <int... Rs>
+struct Radices {
+    static constexpr int S = sizeof...(Rs);
+    static constexpr int R[S] = {Rs...};
+    static constexpr int L = (Rs * ...);
+    // product of the radices before stage s
+    static constexpr int ns(int s) {
+        int p = 1;
+        for (int i = 0; i < s; ++i) p *= R[i];
+        return p;
+    }
+};
+
+template <class RL, int EMAX, int TPLC, bool SPLIT, int s, class T, int OP, bool CONTIG>
+__device__ __forceinline__ void fixed_stages_from(const TileCtx<T, OP, CONTIG> &c, cx<T> *v) {
+    if constexpr (s + 1 < RL::S) {
+        constexpr int Rs = RL::R[s], Rn = RL::R[s + 1], Ns = RL::ns(s), L = RL::L;
+        if constexpr (!SPLIT) {
+            st_write_lds<Rs, EMAX, PART_FULL, L, TPLC, Ns>(c, v);
+            __syncthreads();
+            st_read_lds<Rn, EMAX, PART_FULL, L, TPLC>(c, v);
+            __syncthreads();
+        } else {
+            st_write_lds<Rs, EMAX, PART_RE, L, TPLC, Ns>(c, v);
+            __syncthreads();
+            st_read_lds<Rn, EMAX, PART_RE, L, TPLC>(c, v);
+            __syncthreads();
+            st_write_lds<Rs, EMAX, PART_IM, L, TPLC, Ns>(c, v);
+            __syncthreads();
+            st_read_lds<Rn, EMAX, PART_IM, L, TPLC>(c, v);
+            __syncthreads();
+        }
+        st_compute<Rn, EMAX, L, TPLC, Ns * Rs>(c, v, c.a.sp.tw[s + 1]);
+        fixed_stages_from<RL, EMAX, TPLC, SPLIT, s + 1>(c, v);
+    }
+}
+
+// All stages of a fixed schedule: global -> registers, LDS exchanges, registers -> global.
+template <class RL, int EMAX, int TPLC, bool SPLIT, class T, int OP, bool CONTIG>
+__device__ __forceinline__ void run_fixed_stages(const TileCtx<T, OP, CONTIG> &c) {
+    constexpr int L = RL::L;
+    cx<T> v[EMAX];
+    st_load_global<RL::R[0], EMAX, L, TPLC>(c, v);
+    if constexpr (OP == OP_C2R) __syncthreads();
+    st_compute<RL::R[0], EMAX, L, TPLC, 1>(c, v, 0);
+    fixed_stages_from<RL, EMAX, TPLC, SPLIT, 0>(c, v);
+    st_store_global<RL::R[RL::S - 1], EMAX, L, TPLC>(c, v);
+}
+
 // Runs tile `tile` of pass `a` with the calling workgroup.  Threads with
 // tid >= TPL*TC (a persistent workgroup larger than the tile needs) take part
 // in every barrier but touch no memory.
-template <int EMAX, class T, int OP, bool CONTIG>
+// With RL (a Radices<...> list) the stages are the compile-time schedule RL
+// with TPLC threads per line; otherwise they follow the runtime schedule a.sp.
+struct NoRadices {};
+
+template <int EMAX, class T, int OP, bool CONTIG, bool SPLIT = false, class RL = NoRadices, int TPLC = 0>
 __device__ __forceinline__ void axis_tile(const AxisPass &a, const void *in, void *out, int64_t tile, int tid,
                                           cx<T> *lds) {
-    const int TPL = a.TPL, TC = a.TC;
+    constexpr bool FIXED = !std::is_same<RL, NoRadices>::value;
+    const int TPL = FIXED ? TPLC : a.TPL, TC = a.TC;
     const bool active = tid < TPL * TC;
     int ls, t;
     if constexpr (CONTIG) {
@@ -239,7 +426,7 @@ __device__ __forceinline__ void axis_tile(const AxisPass &a, const void *in, voi
         const int H = sp.L;
         const cx<T> *inc = reinterpret_cast<const cx<T> *>(in);
         for (int k = t; k <= H; k += TPL) {
-            cx<T> x = valid ? inc[ib + (int64_t)k * a.in_sm] : cx<T>{0, 0};
+            cx<T> x = valid ? gload(inc + ib + (int64_t)k * a.in_sm, a.nt) : cx<T>{0, 0};
             if (k == 0 || k == H) x.y = 0;
             lds[c.li(k)] = x;
         }
@@ -249,7 +436,9 @@ __device__ __forceinline__ void axis_tile(const AxisPass &a, const void *in, voi
     if (sp.S == 0) {  // length-1 transform: only the fused load/store maps apply
         if (t == 0) c.store_last(0, c.load0(0));
     }
-    if (sp.S == 1) {
+    if constexpr (FIXED) {
+        run_fixed_stages<RL, EMAX, TPLC, SPLIT>(c);
+    } else if (sp.S == 1) {
         dispatch_stage<SK_ONLY, EMAX>(c, sp.R[0], 1, sp.tw[0]);
     } else if (sp.S > 1) {
         dispatch_stage<SK_FIRST, EMAX>(c, sp.R[0], 1, sp.tw[0]);
@@ -273,7 +462,7 @@ __device__ __forceinline__ void axis_tile(const AxisPass &a, const void *in, voi
             cx<T> zc = conj(lds[c.li(k == 0 ? 0 : H - k)]);
             cx<T> A = zk + zc, B = zk - zc;
             cx<T> X = scl(A + mul_neg_i(cmul(pw[k], B)), (T)0.5 * s);
-            if (valid) outc[ob + (int64_t)k * a.out_sm] = X;
+            if (valid) gstore(outc + ob + (int64_t)k * a.out_sm, X, a.nt);
         }
     }
 }

@@ -177,7 +177,7 @@ int group_launch(const GroupExec *g, void *const *slots, int nslots, hipStream_t
             hipLaunchKernelGGL(k_group<512>, dim3((unsigned)L.nblocks), dim3(512), L.lds, st, a);
         else
             hipLaunchKernelGGL(k_group<256>, dim3((unsigned)L.nblocks), dim3(256), L.lds, st, a);
-        if (hipGetLastError() != hipSuccess) return DFFT_ERR_HIP;
+        if (hipError_t e = hipGetLastError()) return hip_fail(e, "grouped launch");
     }
     return DFFT_OK;
 }
@@ -245,6 +245,7 @@ int dfft_batch_create(dfft_batch **batch, int count, const dfft_batch_desc *desc
         for (int a = 0; a < 3; ++a)
             if (d.n[a] < 1 || d.n[a] > (1 << 20)) return DFFT_ERR_ARG;
     }
+    if (!(flags & DFFT_FLAG_NO_TUNE)) wisdom_autoload();
     dfft_batch *b = new (std::nothrow) dfft_batch();
     if (!b) return DFFT_ERR_OOM;
     b->count = count;
@@ -281,6 +282,7 @@ int dfft_batch_create(dfft_batch **batch, int count, const dfft_batch_desc *desc
                 a.nlog = (int)n[0];
                 a.scale = 1.0;
                 set_geometry(a, 0, n, nc);
+                a.nt = 1;
                 pp.alg_bytes = N * es_c / 2 + NC * es_c;
             } else {
                 const int64_t *gsz = d.kind == DFFT_R2C ? nc : n;
@@ -291,13 +293,14 @@ int dfft_batch_create(dfft_batch **batch, int count, const dfft_batch_desc *desc
                 a.inverse = inv ? 1 : 0;
                 a.scale = (inv && norm) ? 1.0 / L : 1.0;
                 set_geometry(a, ax, gsz, gsz);
+                a.nt = 1;
                 pp.alg_bytes = 2.0 * (d.kind == DFFT_R2C ? NC : N) * es_c;
             }
             b->alg_bytes += pp.alg_bytes;
             Tuning tn;
             const Tuning *tune = nullptr;
             if (!(flags & DFFT_FLAG_NO_TUNE) && tuning_lookup(pp, tn) && tn.tpl <= 512) tune = &tn;
-            rc = schedule_pass(pp, tune, 512, 65536);
+            rc = schedule_pass(pp, tune, 512, 65536, 256, false);
             if (rc != DFFT_OK) break;
             (prec ? tt64 : tt32).add(pp.ap);
             lv[i].push_back(pp);

@@ -9,6 +9,9 @@
 
 #include "dfft_axis.hpp"
 #include "dfft_kernels.hpp"
+#include "dfft_plan.hpp"
+
+#include <vector>
 
 namespace dfft {
 
@@ -30,11 +33,70 @@ using axis_kernel_t = void (*)(AxisPass);
      (axis_kernel_t)k_axis<T, OP_C2R_FULL, CONTIG, 8>, (axis_kernel_t)k_axis<T, OP_DCT2, CONTIG, 8>, \
      (axis_kernel_t)k_axis<T, OP_DCT3, CONTIG, 8>}
 
-// [precision][contig][op]
+// runtime-schedule kernels: [precision][contig][op]
 static axis_kernel_t g_axis_kernels[2][2][OP_COUNT] = {
     {DFFT_OPS(float, false), DFFT_OPS(float, true)},
     {DFFT_OPS(double, false), DFFT_OPS(double, true)},
 };
+
+// ---------------------------------------------------------------------------
+// compile-time schedules of the hot C2C lengths: every stage unrolled with
+// constant radix, length and threads per line
+// ---------------------------------------------------------------------------
+template <class T, bool CONTIG, int EMAX, bool SPLIT, class RL, int TPLC>
+__global__ __launch_bounds__(1024) void k_axis_fixed(AxisPass a) {
+    extern __shared__ __align__(16) unsigned char dfft_smem[];
+    axis_tile<EMAX, T, OP_C2C, CONTIG, SPLIT, RL, TPLC>(a, a.in, a.out, (int64_t)blockIdx.x, (int)threadIdx.x,
+                                                        reinterpret_cast<cx<T> *>(dfft_smem));
+}
+
+struct FixedEntry {
+    int prec, contig, variant, L, tpl, S;
+    int R[MAX_STAGES];
+    axis_kernel_t fn;
+};
+
+#define DFFT_FIXED(T, P, RL, ...)                                                                          \
+    {P, 1, 1, RL::L, RL::L / 8, RL::S, {__VA_ARGS__}, (axis_kernel_t)k_axis_fixed<T, true, 8, false, RL, RL::L / 8>},  \
+    {P, 0, 1, RL::L, RL::L / 8, RL::S, {__VA_ARGS__}, (axis_kernel_t)k_axis_fixed<T, false, 8, false, RL, RL::L / 8>}, \
+    {P, 0, 2, RL::L, RL::L / 16, RL::S, {__VA_ARGS__}, (axis_kernel_t)k_axis_fixed<T, false, 16, true, RL, RL::L / 16>}
+
+using R64 = Radices<8, 8>;
+using R128 = Radices<8, 8, 2>;
+using R256 = Radices<8, 8, 4>;
+using R512 = Radices<8, 8, 8>;
+using R1024 = Radices<8, 8, 8, 2>;
+
+static const FixedEntry g_fixed[] = {
+    DFFT_FIXED(double, 1, R64, 8, 8),        DFFT_FIXED(double, 1, R128, 8, 8, 2),
+    DFFT_FIXED(double, 1, R256, 8, 8, 4),    DFFT_FIXED(double, 1, R512, 8, 8, 8),
+    DFFT_FIXED(double, 1, R1024, 8, 8, 8, 2), DFFT_FIXED(float, 0, R64, 8, 8),
+    DFFT_FIXED(float, 0, R128, 8, 8, 2),     DFFT_FIXED(float, 0, R256, 8, 8, 4),
+    DFFT_FIXED(float, 0, R512, 8, 8, 8),     DFFT_FIXED(float, 0, R1024, 8, 8, 8, 2),
+};
+
+const FixedEntry *fixed_entry(int prec, int contig, int variant, int op, int L) {
+    if (op != OP_C2C || variant < 1) return nullptr;
+    for (const auto &e : g_fixed)
+        if (e.prec == (prec ? 1 : 0) && e.contig == (contig ? 1 : 0) && e.variant == variant && e.L == L) return &e;
+    return nullptr;
+}
+
+bool fixed_schedule(int prec, int contig, int variant, int op, int L, std::vector<int> &radices, int &tpl) {
+    const FixedEntry *e = fixed_entry(prec, contig, variant, op, L);
+    if (!e) return false;
+    radices.assign(e->R, e->R + e->S);
+    tpl = e->tpl;
+    return true;
+}
+
+static axis_kernel_t axis_kernel(const AxisPass &a, int prec) {
+    if (a.variant >= 1) {
+        const FixedEntry *e = fixed_entry(prec, a.contig, a.variant, a.op, a.sp.L);
+        return e ? e->fn : nullptr;
+    }
+    return g_axis_kernels[prec ? 1 : 0][a.contig ? 1 : 0][a.op];
+}
 
 static int g_lds_limit = 0;
 
@@ -52,7 +114,8 @@ int axis_lds_limit() {
 
 hipError_t launch_axis(const AxisPass &a, int prec, int64_t nblocks, int threads, size_t lds,
                        hipStream_t stream) {
-    axis_kernel_t k = g_axis_kernels[prec ? 1 : 0][a.contig ? 1 : 0][a.op];
+    axis_kernel_t k = axis_kernel(a, prec);
+    if (!k) return hipErrorInvalidDeviceFunction;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
@@ -62,7 +125,7 @@ hipError_t launch_axis(const AxisPass &a, int prec, int64_t nblocks, int threads
 }
 
 void *axis_kernel_fn(const AxisPass &a, int prec) {
-    return (void *)g_axis_kernels[prec ? 1 : 0][a.contig ? 1 : 0][a.op];
+    return (void *)axis_kernel(a, prec);
 }
 
 // ---------------------------------------------------------------------------

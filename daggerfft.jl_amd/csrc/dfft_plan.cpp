@@ -27,6 +27,14 @@
 
 namespace dfft {
 
+static thread_local char g_last_error[256] = "";
+
+int hip_fail(int hip_status, const char *what) {
+    snprintf(g_last_error, sizeof g_last_error, "%s: %s (hip status %d)", what,
+             hipGetErrorString((hipError_t)hip_status), hip_status);
+    return DFFT_ERR_HIP;
+}
+
 // ---------------------------------------------------------------------------
 // twiddles: exp(-2 pi i k / n) evaluated in long double with octant folding,
 // rounded once to the pass precision
@@ -124,7 +132,7 @@ int TwiddleTable::upload(void **d_ptr, size_t *bytes) const {
         *d_ptr = nullptr;
         return DFFT_ERR_OOM;
     }
-    if (hipMemcpy(*d_ptr, host.data(), nb, hipMemcpyHostToDevice) != hipSuccess) return DFFT_ERR_HIP;
+    if (hipError_t e = hipMemcpy(*d_ptr, host.data(), nb, hipMemcpyHostToDevice)) return hip_fail(e, "twiddle upload");
     *bytes = nb;
     return DFFT_OK;
 }
@@ -152,20 +160,39 @@ static bool tpl_ok(int L, const std::vector<int> &R, int emax, int tpl) {
 // ---------------------------------------------------------------------------
 // schedule of one pass: radices + launch shape
 // ---------------------------------------------------------------------------
-int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_lds, int contig_threads) {
+int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_lds, int contig_threads,
+                  bool allow_split) {
     AxisPass &a = pp.ap;
     const int L = a.sp.L;
     const int es = pp.prec ? 16 : 8;
+    // variant 1/2: a compile-time schedule of this length (2 also halves the
+    // LDS by exchanging real and imaginary parts in two rounds; strided only)
+    // default: the split compile-time schedule for strided FP64 lines of >= 256
+    // points (two tiles per CU fit in LDS), else the compile-time schedule
+    int want = tune ? tune->variant : ((!a.contig && pp.prec && L >= 256) ? 2 : 1);
+    if (want == 2 && (!allow_split || a.contig)) want = 1;
     std::vector<int> R;
-    if (tune && !tune->radices.empty()) R = tune->radices;
-    else if (!factorize(L, R)) return DFFT_ERR_UNSUPPORTED_SIZE;
+    int fixed_tpl = 0;
+    a.variant = 0;
+    if (want >= 1 && fixed_schedule(pp.prec, a.contig, want, a.op, L, R, fixed_tpl)) {
+        a.variant = want;
+    } else if (want == 2 && fixed_schedule(pp.prec, a.contig, 1, a.op, L, R, fixed_tpl)) {
+        a.variant = 1;
+    } else if (tune && !tune->radices.empty()) {
+        R = tune->radices;
+    } else if (!factorize(L, R)) {
+        return DFFT_ERR_UNSUPPORTED_SIZE;
+    }
     int prod = 1;
     for (int r : R) prod *= r;
     if (prod != L || (int)R.size() > MAX_STAGES) return DFFT_ERR_UNSUPPORTED_SIZE;
     a.sp.S = (int)R.size();
     for (int s = 0; s < a.sp.S; ++s) a.sp.R[s] = R[s];
-    const int emax = EMAX;
-    int tpl = (tune && tune->tpl > 0 && tpl_ok(L, R, emax, tune->tpl)) ? tune->tpl : default_tpl(L, R, emax, 8);
+    const int emax = a.variant == 2 ? 16 : EMAX;
+    const int lds_es = a.variant == 2 ? es / 2 : es;
+    int tpl = a.variant ? fixed_tpl
+                        : (tune && tune->tpl > 0 && tpl_ok(L, R, emax, tune->tpl)) ? tune->tpl
+                                                                                    : default_tpl(L, R, emax, emax);
     if (L == 1) tpl = 1;
     if (tpl > max_threads) return DFFT_ERR_UNSUPPORTED_SIZE;
     a.TPL = tpl;
@@ -174,27 +201,31 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
     if (a.contig) {
         a.pad_shift = (tune && tune->pad_shift > 0) ? tune->pad_shift : (pp.prec ? 3 : 4);
         a.lds_pitch = rows + (rows >> a.pad_shift) + 1;
-        int tc = (tune && tune->tc > 0) ? tune->tc : std::max(1, contig_threads / tpl);
+        // 128-thread workgroups by default: many small tiles in flight per CU
+        int tc = (tune && tune->tc > 0) ? tune->tc : std::max(1, (contig_threads > 256 ? contig_threads : 128) / tpl);
         while (tc > 1 && (int64_t)tc * tpl > max_threads) --tc;
         const int contig_lds = contig_threads > 256 ? lds_limit : std::min(lds_limit, 65536);
-        while (tc > 1 && (size_t)tc * a.lds_pitch * es > (size_t)contig_lds) --tc;
-        if ((size_t)a.lds_pitch * es > (size_t)lds_limit) return DFFT_ERR_UNSUPPORTED_SIZE;
+        while (tc > 1 && (size_t)tc * a.lds_pitch * lds_es > (size_t)contig_lds) --tc;
+        if ((size_t)a.lds_pitch * lds_es > (size_t)lds_limit) return DFFT_ERR_UNSUPPORTED_SIZE;
         a.TC = tc;
         pp.threads = tpl * tc;
-        pp.lds = (size_t)tc * a.lds_pitch * es;
+        pp.lds = (size_t)tc * a.lds_pitch * lds_es;
         pp.nblocks = (a.nlines + tc - 1) / tc;
     } else {
         a.pad_shift = 30;
-        int tc = (tune && tune->tc > 0) ? tune->tc : (pp.prec ? 16 : 32);
+        // 256-byte (FP64) / 512-byte rows; the split schedule doubles the row over outer slices
+        int tc = (tune && tune->tc > 0) ? tune->tc
+                                        : (a.variant == 2 ? (a.nouter > 1 ? 32 : 16) : (pp.prec ? 16 : 32));
         int64_t cols = a.nlines;
         while (tc > 1 && tc / 2 >= cols) tc /= 2;
         while (tc > 1 && (int64_t)tc * tpl > max_threads) tc /= 2;
-        while (tc > 1 && (size_t)rows * tc * es > (size_t)lds_limit) tc /= 2;
-        if ((size_t)rows * tc * es > (size_t)lds_limit || (int64_t)tc * tpl > max_threads) return DFFT_ERR_UNSUPPORTED_SIZE;
+        while (tc > 1 && (size_t)rows * tc * lds_es > (size_t)lds_limit) tc /= 2;
+        if ((size_t)rows * tc * lds_es > (size_t)lds_limit || (int64_t)tc * tpl > max_threads)
+            return DFFT_ERR_UNSUPPORTED_SIZE;
         a.TC = tc;
         a.lds_pitch = tc;
         pp.threads = tpl * tc;
-        pp.lds = (size_t)rows * tc * es;
+        pp.lds = (size_t)rows * tc * lds_es;
         pp.nblocks = ((a.nlines + tc - 1) / tc) * a.nouter;
     }
     if (pp.nblocks > 0x7fffffffLL) return DFFT_ERR_UNSUPPORTED_SIZE;
@@ -326,6 +357,7 @@ int build_plan(dfft_plan *p) {
         a.scale = scale;
         a.nlog = nlog;
         a.sp.L = L;
+        a.nt = 1;  // every element is read once and written once per pass
         set_geometry(a, axis, nin, nout);
         p->passes.push_back(pp);
     };
@@ -333,6 +365,13 @@ int build_plan(dfft_plan *p) {
     if (p->kind == DFFT_C2C) {
         p->in_bytes = p->out_bytes = (size_t)N * es_c;
         if (inv) std::reverse(order.begin(), order.end());
+        // The 1D transforms along distinct axes commute.  The first pass is the
+        // out-of-place one, and out-of-place is cheapest on the contiguous axis,
+        // so it goes first unless the reference's stage order is requested.
+        if (!(p->flags & DFFT_FLAG_STAGE_ORDER)) {
+            auto it = std::find(order.begin(), order.end(), 0);
+            if (it != order.end()) std::rotate(order.begin(), it, it + 1);
+        }
         for (size_t i = 0; i < order.size(); ++i) {
             const int ax = order[i];
             const int L = (int)n[ax];
@@ -397,7 +436,8 @@ int build_plan(dfft_plan *p) {
     }
     // Slab: the 2D stage (dims[0], dims[1]) is software-pipelined over chunks
     // of z-planes; the dims[2] pass follows as one launch
-    const bool slab_ok = p->decomp == DFFT_SLAB && p->rank == 3 && p->ndims == 3 && n[2] > 1 &&
+    const bool slab_ok = p->decomp == DFFT_SLAB && (p->flags & DFFT_FLAG_PLANE_PIPELINE) && p->rank == 3 &&
+                         p->ndims == 3 && n[2] > 1 &&
                          p->dims[2] == 2 && p->kind != DFFT_R2R && !(p->flags & DFFT_FLAG_GRAPH) &&
                          std::all_of(p->passes.begin(), p->passes.end(), [](const PassPlan &q) {
                              // ops the grouped kernel dispatches
@@ -438,9 +478,10 @@ int build_plan(dfft_plan *p) {
                 const Tuning *tune = nullptr;
                 Tuning t;
                 if (!(p->flags & DFFT_FLAG_NO_TUNE) && tuning_lookup(q, t)) tune = &t;
-                int rc2 = schedule_pass(q, tune, 1024, 0, 1024);
+                int rc2 = schedule_pass(q, tune, 1024, 0, 1024, false);
                 if (rc2 != DFFT_OK) return rc2;
                 q.alg_bytes = full.alg_bytes * (double)planes / (double)n[2];
+                q.ap.nt = 0;  // the next launch re-reads this chunk from the Infinity Cache
                 tt.add(q.ap);
                 (which == 0 ? cpa : cpb).push_back(ChunkPass{q, k0, planes});
             }
@@ -512,7 +553,7 @@ static int run_passes(const dfft_plan *p, const void *d_in, void *d_out, void *w
         AxisPass a = pp.ap;
         a.in = resolve(pp.src, d_in, d_out, work);
         a.out = resolve(pp.dst, d_in, d_out, work);
-        if (launch_axis(a, pp.prec, pp.nblocks, pp.threads, pp.lds, st) != hipSuccess) return DFFT_ERR_HIP;
+        if (hipError_t e = launch_axis(a, pp.prec, pp.nblocks, pp.threads, pp.lds, st)) return hip_fail(e, "axis pass launch");
     }
     return DFFT_OK;
 }
@@ -605,7 +646,11 @@ int dfft_plan_create(dfft_plan **plan, int rank, const int64_t *n, int kind, int
     }
     const int64_t total = p->n[0] * p->n[1] * p->n[2];
     if (total > (1LL << 40)) { delete p; return DFFT_ERR_ARG; }
-    if (hipGetDevice(&p->device) != hipSuccess) { delete p; return DFFT_ERR_HIP; }
+    if (hipError_t e = hipGetDevice(&p->device)) {
+        delete p;
+        return hip_fail(e, "hipGetDevice");
+    }
+    if (!(flags & DFFT_FLAG_NO_TUNE)) wisdom_autoload();
     int rc;
     try {
         rc = build_plan(p);
@@ -674,8 +719,8 @@ int dfft_execute_pass(const dfft_plan *p, int pass, const void *d_in, void *d_ou
     AxisPass a = pp.ap;
     a.in = resolve(pp.src, d_in, d_out, work);
     a.out = resolve(pp.dst, d_in, d_out, work);
-    return launch_axis(a, pp.prec, pp.nblocks, pp.threads, pp.lds, (hipStream_t)stream) == hipSuccess ? DFFT_OK
-                                                                                                     : DFFT_ERR_HIP;
+    hipError_t e = launch_axis(a, pp.prec, pp.nblocks, pp.threads, pp.lds, (hipStream_t)stream);
+    return e == hipSuccess ? DFFT_OK : hip_fail(e, "axis pass launch");
 }
 
 int dfft_plan_destroy(dfft_plan *p) {
@@ -694,14 +739,26 @@ int dfft_plan_describe(const dfft_plan *p, int pass, char *buf, size_t buflen) {
     if (!p || !buf || pass < 0 || pass >= (int)p->passes.size()) return DFFT_ERR_ARG;
     const PassPlan &pp = p->passes[pass];
     const AxisPass &a = pp.ap;
-    std::string rad;
-    for (int s = 0; s < a.sp.S; ++s) rad += (s ? "x" : "") + std::to_string(a.sp.R[s]);
+    std::string rad, rl;
+    for (int s = 0; s < a.sp.S; ++s) {
+        rad += (s ? "x" : "") + std::to_string(a.sp.R[s]);
+        rl += (s ? "," : "") + std::to_string(a.sp.R[s]);
+    }
     if (rad.empty()) rad = "1";
+    // the kernel's name as profilers print it (spaces removed)
+    const char *T = pp.prec ? "double" : "float";
+    std::string kname;
+    if (a.variant >= 1)
+        kname = std::string("k_axis_fixed<") + T + "," + (a.contig ? "true" : "false") + "," +
+                (a.variant == 2 ? "16,true" : "8,false") + ",dfft::Radices<" + rl + ">," + std::to_string(a.TPL) + ">";
+    else
+        kname = std::string("k_axis<") + T + "," + std::to_string(a.op) + "," + (a.contig ? "true" : "false") + ",8>";
     snprintf(buf, buflen,
              "{\"pass\":%d,\"axis\":%d,\"op\":\"%s\",\"L\":%d,\"nlog\":%d,\"radices\":\"%s\",\"TPL\":%d,\"TC\":%d,"
-             "\"contig\":%d,\"threads\":%d,\"blocks\":%lld,\"lds\":%zu,\"bytes\":%.0f,\"prec\":\"%s\"}",
-             pass, pp.axis, op_name(a.op), a.sp.L, a.nlog, rad.c_str(), a.TPL, a.TC, a.contig, pp.threads,
-             (long long)pp.nblocks, pp.lds, pp.alg_bytes, pp.prec ? "f64" : "f32");
+             "\"contig\":%d,\"variant\":%d,\"threads\":%d,\"blocks\":%lld,\"lds\":%zu,\"bytes\":%.0f,\"prec\":\"%s\","
+             "\"kernel\":\"%s\"}",
+             pass, pp.axis, op_name(a.op), a.sp.L, a.nlog, rad.c_str(), a.TPL, a.TC, a.contig, a.variant, pp.threads,
+             (long long)pp.nblocks, pp.lds, pp.alg_bytes, pp.prec ? "f64" : "f32", kname.c_str());
     return DFFT_OK;
 }
 
@@ -719,6 +776,8 @@ int dfft_plan_algorithmic_bytes(const dfft_plan *p, double *bytes) {
     *bytes = b;
     return DFFT_OK;
 }
+
+const char *dfft_last_error(void) { return g_last_error; }
 
 const char *dfft_strerror(int s) {
     switch (s) {

@@ -12,7 +12,7 @@
 
 namespace dfft {
 
-constexpr int EMAX = 8;  // complex elements a thread may hold per stage
+constexpr int EMAX = 8;  // complex values a thread holds per stage (variant 0; variant 1 holds 16)
 
 enum BufId : int { BUF_IN = 0, BUF_OUT = 1, BUF_WORK = 2 };
 
@@ -33,12 +33,16 @@ struct Tuning {
     int tpl = 0;
     int tc = 0;
     int pad_shift = 0;
+    int variant = 0;  // see AxisPass::variant
 };
 
+// Records `what` and HIP's message as this thread's last error (dfft_last_error)
+// and returns DFFT_ERR_HIP.
+int hip_fail(int hip_status, const char *what);
 bool factorize(int L, std::vector<int> &R);
 void set_geometry(AxisPass &a, int axis, const int64_t nin[3], const int64_t nout[3]);
 int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads = 1024, size_t max_lds = 0,
-                  int contig_threads = 256);
+                  int contig_threads = 256, bool allow_split = true);
 
 // Stage twiddle + pre/post blocks of every distinct (op, L, radices) of one
 // precision, in one device table.
@@ -50,6 +54,7 @@ struct TwiddleTable {
     int upload(void **d_ptr, size_t *bytes) const;
 };
 bool tuning_lookup(const PassPlan &pp, Tuning &t);
+void wisdom_autoload();
 std::string tuning_key(const PassPlan &pp);
 void cis_ld(int64_t k, int64_t n, long double *c, long double *s);
 

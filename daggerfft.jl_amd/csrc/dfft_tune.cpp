@@ -10,7 +10,10 @@
 // to and loaded from a text file (one entry per line).
 #include <hip/hip_runtime.h>
 
+#include <dlfcn.h>
+
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -30,6 +33,8 @@ namespace dfft {
 
 static std::mutex g_mu;
 static std::unordered_map<std::string, Tuning> g_cache;
+static std::once_flag g_wisdom_once;
+static bool g_wisdom_done = false;  // guarded by g_mu
 
 static int size_class(const AxisPass &a) {
     const double lines = (double)a.nlines * (double)a.nouter;
@@ -38,8 +43,44 @@ static int size_class(const AxisPass &a) {
 
 std::string tuning_key(const PassPlan &pp) {
     const AxisPass &a = pp.ap;
+    // strided passes over outer slices (axis 1) and over whole planes (axis 2) differ
+    const int layout = a.contig ? 1 : (a.nouter > 1 ? 2 : 0);
     return std::to_string(pp.prec) + "/" + std::to_string(a.op) + "/" + std::to_string(a.sp.L) + "/" +
-           std::to_string(a.contig) + "/" + std::to_string(size_class(a));
+           std::to_string(layout) + "/" + std::to_string(size_class(a));
+}
+
+static int load_file(const char *path);
+
+// The schedule cache shipped beside the library (the reference imports its
+// FFTW wisdom file at module init, src/fft.jl:116-120): $DFFT_WISDOM if set
+// ("off" disables), else <library dir>/wisdom/<gfx arch>.txt.  A missing file
+// is not an error.
+void wisdom_autoload() {
+    std::call_once(g_wisdom_once, [] {
+        {
+            std::lock_guard<std::mutex> g(g_mu);
+            if (g_wisdom_done) return;
+            g_wisdom_done = true;
+        }
+        std::string path;
+        if (const char *e = getenv("DFFT_WISDOM")) {
+            if (!strcmp(e, "off") || !*e) return;
+            path = e;
+        } else {
+            Dl_info info;
+            if (!dladdr((void *)&wisdom_autoload, &info) || !info.dli_fname) return;
+            std::string lib = info.dli_fname;
+            const size_t slash = lib.rfind('/');
+            const std::string dir = slash == std::string::npos ? "." : lib.substr(0, slash);
+            int dev = 0;
+            hipDeviceProp_t prop;
+            if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return;
+            std::string arch = prop.gcnArchName;
+            arch = arch.substr(0, arch.find(':'));
+            path = dir + "/wisdom/" + arch + ".txt";
+        }
+        (void)load_file(path.c_str());
+    });
 }
 
 bool tuning_lookup(const PassPlan &pp, Tuning &t) {
@@ -92,15 +133,31 @@ static std::vector<std::vector<int>> radix_candidates(int L) {
 static std::vector<Tuning> candidates(const PassPlan &pp, int level, int max_threads) {
     const AxisPass &a = pp.ap;
     std::vector<Tuning> out;
-    auto rads = radix_candidates(a.sp.L);
-    if (level <= 1 && rads.size() > 4) rads.resize(4);
-    std::vector<int> targets = {8, 4};
-    std::vector<int> tcs = a.contig ? std::vector<int>{1, 2, 4, 8, 16} : std::vector<int>{4, 8, 16, 32};
+    std::vector<int> tcs = a.contig ? std::vector<int>{1, 2, 4, 8, 16} : std::vector<int>{4, 8, 16, 32, 64};
     std::vector<int> pads = a.contig ? (level >= 2 ? std::vector<int>{3, 4, 5} : std::vector<int>{3, 4})
                                      : std::vector<int>{0};
+    // compile-time schedules of this length: only the tile shape varies
+    for (int var : {1, 2}) {
+        std::vector<int> R;
+        int tpl = 0;
+        if (!fixed_schedule(pp.prec, a.contig, var, a.op, a.sp.L, R, tpl)) continue;
+        for (int tc : tcs) {
+            if (tpl * tc > max_threads) continue;
+            for (int ps : pads) {
+                Tuning t;
+                t.variant = var;
+                t.tc = tc;
+                t.pad_shift = ps;
+                out.push_back(t);
+            }
+        }
+    }
+    // runtime schedules: radix orders x threads per line x tile shape
+    auto rads = radix_candidates(a.sp.L);
+    if (level <= 1 && rads.size() > 4) rads.resize(4);
     for (const auto &R : rads) {
         std::set<int> tpls;
-        for (int e : targets) {
+        for (int e : {8, 4}) {
             int tpl = std::max(1, (a.sp.L + e - 1) / e);
             for (;;) {  // smallest thread count >= L/e that holds every stage in EMAX registers
                 bool ok = true;
@@ -120,6 +177,7 @@ static std::vector<Tuning> candidates(const PassPlan &pp, int level, int max_thr
                     t.tpl = tpl;
                     t.tc = tc;
                     t.pad_shift = ps;
+                    t.variant = 0;
                     out.push_back(t);
                 }
             }
@@ -159,9 +217,13 @@ int dfft_autotune(int rank, const int64_t *n, int kind, int sign, const int *r2r
         if (nb) (void)launch_fill(buf[b], (int64_t)(nb / (precision ? 8 : 4)), precision, 7 + b, st);
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (rc == DFFT_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = DFFT_ERR_HIP;
+    if (rc == DFFT_OK) {
+        if (hipError_t e = hipEventCreate(&e0)) rc = hip_fail(e, "hipEventCreate");
+        else if (hipError_t e2 = hipEventCreate(&e1)) rc = hip_fail(e2, "hipEventCreate");
+    }
     std::set<std::string> done;
     const int reps = level >= 2 ? 5 : 3;
+    const bool verbose = getenv("DFFT_TUNE_VERBOSE") != nullptr;
     for (int i = 0; rc == DFFT_OK && i < plan_pass_count(p); ++i) {
         PassPlan base = plan_pass(p, i);
         const std::string key = tuning_key(base);
@@ -194,6 +256,13 @@ int dfft_autotune(int rank, const int64_t *n, int kind, int sign, const int *r2r
                 ok = ok && hipEventSynchronize(e1) == hipSuccess;
             }
             float ms = 0;
+            if (ok && verbose && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) {
+                std::string rad;
+                for (int r : cand.radices) rad += (rad.empty() ? "" : "x") + std::to_string(r);
+                fprintf(stderr, "tune %s radices=%s var=%d tpl=%d tc=%d pad=%d threads=%d lds=%zu: %.4f ms %.1f GB/s\n",
+                        key.c_str(), rad.c_str(), pp.ap.variant, pp.ap.TPL, pp.ap.TC, pp.ap.pad_shift, pp.threads, pp.lds,
+                        ms / reps, pp.alg_bytes / (ms / reps * 1e-3) / 1e9);
+            }
             if (ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms / reps < best) {
                 best = ms / reps;
                 best_t = cand;
@@ -217,18 +286,26 @@ int dfft_tune_save(const char *path) {
     std::ofstream f(path);
     if (!f) return DFFT_ERR_ARG;
     std::lock_guard<std::mutex> g(g_mu);
-    f << "# daggerfft_hip schedule cache: key radices tpl tc pad_shift\n";
+    f << "# daggerfft_hip schedule cache: key radices tpl tc pad_shift variant\n";
     for (const auto &kv : g_cache) {
         f << kv.first << " ";
         for (size_t i = 0; i < kv.second.radices.size(); ++i) f << (i ? "," : "") << kv.second.radices[i];
         if (kv.second.radices.empty()) f << "-";
-        f << " " << kv.second.tpl << " " << kv.second.tc << " " << kv.second.pad_shift << "\n";
+        f << " " << kv.second.tpl << " " << kv.second.tc << " " << kv.second.pad_shift << " " << kv.second.variant
+          << "\n";
     }
     return f.good() ? DFFT_OK : DFFT_ERR_ARG;
 }
 
 int dfft_tune_load(const char *path) {
     if (!path) return DFFT_ERR_ARG;
+    return load_file(path);
+}
+
+}  // extern "C"
+
+namespace dfft {
+static int load_file(const char *path) {
     std::ifstream f(path);
     if (!f) return DFFT_ERR_ARG;
     std::string line;
@@ -238,6 +315,8 @@ int dfft_tune_load(const char *path) {
         std::string key, rad;
         Tuning t;
         if (!(ss >> key >> rad >> t.tpl >> t.tc >> t.pad_shift)) return DFFT_ERR_ARG;
+        if (!(ss >> t.variant)) t.variant = 0;
+        if (t.variant < 0 || t.variant > 2) return DFFT_ERR_ARG;
         if (rad != "-") {
             std::stringstream rs(rad);
             std::string tok;
@@ -254,9 +333,14 @@ int dfft_tune_load(const char *path) {
     return DFFT_OK;
 }
 
+}  // namespace dfft
+
+extern "C" {
+
 int dfft_tune_clear(void) {
     std::lock_guard<std::mutex> g(g_mu);
     g_cache.clear();
+    g_wisdom_done = true;  // a cleared cache stays empty
     return DFFT_OK;
 }
 

@@ -251,9 +251,9 @@ class Plan:
         n = ctypes.c_int()
         check(lib().dfft_plan_num_passes(self._h, ctypes.byref(n)))
         out = []
-        buf = ctypes.create_string_buffer(512)
+        buf = ctypes.create_string_buffer(1024)
         for i in range(n.value):
-            check(lib().dfft_plan_describe(self._h, i, buf, 512))
+            check(lib().dfft_plan_describe(self._h, i, buf, 1024))
             out.append(json.loads(buf.value.decode()))
         return out
 
@@ -314,6 +314,7 @@ def autotune(shape, kind=L.C2C, sign=-1, precision=L.F64, decomp=Pencil(), dims=
     created afterwards use it.  Cached plans of the same shape are dropped."""
     dims = tuple(range(1, len(shape) + 1)) if dims is None else tuple(dims)
     kinds = L.i32(r2r_kinds) if r2r_kinds is not None else None
+    _torch().cuda.init()
     check(lib().dfft_autotune(len(shape), L.i64(shape), kind, sign, kinds, precision, decomp.code,
                               L.i32([d - 1 for d in dims]), len(dims), flags, level, _stream_handle(stream)),
           "dfft_autotune")

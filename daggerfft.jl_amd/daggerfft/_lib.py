@@ -31,6 +31,8 @@ FLAG_NORMALIZE_R2R = 0x2
 FLAG_DESTROY_INPUT = 0x4
 FLAG_NO_TUNE = 0x8
 FLAG_GRAPH = 0x10
+FLAG_STAGE_ORDER = 0x20
+FLAG_PLANE_PIPELINE = 0x40
 
 _lib = None
 _lock = threading.Lock()
@@ -40,6 +42,8 @@ class DFFTError(RuntimeError):
     def __init__(self, status, where=""):
         self.status = status
         msg = strerror(status) if _lib is not None else str(status)
+        if _lib is not None and status == DFFT_ERR_HIP:
+            msg += " [" + _lib.dfft_last_error().decode() + "]"
         super().__init__("%s: %s (status %d)" % (where or "libdaggerfft_hip", msg, status))
 
 
@@ -92,6 +96,8 @@ def _declare(L):
         f.restype = ctypes.c_int
     L.dfft_strerror.argtypes = [ctypes.c_int]
     L.dfft_strerror.restype = ctypes.c_char_p
+    L.dfft_last_error.argtypes = []
+    L.dfft_last_error.restype = ctypes.c_char_p
 
 
 def lib():

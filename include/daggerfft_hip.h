@@ -86,6 +86,11 @@ extern "C" {
 #define DFFT_FLAG_DESTROY_INPUT  0x4u  /* C2R may overwrite its input (no workspace needed) */
 #define DFFT_FLAG_NO_TUNE        0x8u  /* skip the autotuner cache; use the heuristic schedule */
 #define DFFT_FLAG_GRAPH          0x10u /* replay the pass list as a hipGraph */
+#define DFFT_FLAG_STAGE_ORDER    0x20u /* C2C: run the axis passes in the reference's stage order
+                                          (dims[0..] forward, reversed inverse); by default the
+                                          contiguous axis runs first -- same result, the passes commute */
+#define DFFT_FLAG_PLANE_PIPELINE 0x40u /* Slab: software-pipeline the 2D stage over z-plane chunks
+                                          (grouped launches re-reading each chunk from the Infinity Cache) */
 
 typedef struct dfft_plan dfft_plan;
 
@@ -137,8 +142,8 @@ int dfft_execute_pass(const dfft_plan *plan, int pass, const void *d_in, void *d
 int dfft_plan_num_passes(const dfft_plan *plan, int *npasses);
 int dfft_plan_describe(const dfft_plan *plan, int pass, char *buf, size_t buflen);
 
-/* How dfft_execute runs the plan: one launch per axis pass, or (Slab plans of
- * rank-3 C2C/R2C/C2R arrays with dims[2] == 2) a software pipeline over chunks
+/* How dfft_execute runs the plan: one launch per axis pass, or (Slab plans with
+ * DFFT_FLAG_PLANE_PIPELINE of rank-3 C2C/R2C/C2R arrays with dims[2] == 2) a software pipeline over chunks
  * of z-planes -- launch i runs the dims[0] pass of chunk i together with the
  * dims[1] pass of chunk i-1, which is re-read while still in the Infinity
  * Cache -- followed by the dims[2] pass.  `launches`: kernel launches per
@@ -151,6 +156,8 @@ int dfft_plan_schedule(const dfft_plan *plan, int *schedule, int *launches);
 int dfft_plan_algorithmic_bytes(const dfft_plan *plan, double *bytes);
 
 const char *dfft_strerror(int status);
+/* The calling thread's last DFFT_ERR_HIP cause: the failing HIP call and HIP's message. */
+const char *dfft_last_error(void);
 int dfft_version(int *major, int *minor);
 
 /*

@@ -114,13 +114,14 @@ def test_c2c_inplace_and_graph(gpu):
 
 @pytest.mark.parametrize("shape", [(64, 48, 40), (30, 14, 9), (128, 128, 128)])
 def test_slab_plane_pipeline_matches_pencil(gpu, shape):
-    """Slab plans software-pipeline the 2D stage over chunks of z-planes in
-    grouped launches; results equal the per-pass Pencil schedule and the oracle."""
+    """Slab plans with the plane-pipeline flag software-pipeline the 2D stage
+    over chunks of z-planes in grouped launches; results equal the per-pass
+    Pencil schedule and the oracle."""
     import daggerfft as D
     from daggerfft import _lib as L
     a = O.fill(17, shape, np.complex128)
     A = _dev(a)
-    ps = D.Plan(shape, L.C2C, -1, L.F64, D.Slab(), (1, 2, 3))
+    ps = D.Plan(shape, L.C2C, -1, L.F64, D.Slab(), (1, 2, 3), flags=L.FLAG_PLANE_PIPELINE)
     assert ps.schedule()[0] == "plane_pipeline"
     pp = D.Plan(shape, L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3))
     assert pp.schedule()[0] == "passes"
@@ -128,11 +129,12 @@ def test_slab_plane_pipeline_matches_pencil(gpu, shape):
     Fp = pp.execute(A, A.similar())
     assert O.rel_l2(Fs.to_host(), Fp.to_host()) < 1e-15
     assert O.rel_l2(Fs.to_host(), O.fftn(a)) <= TOL64
-    pi = D.Plan(shape, L.C2C, +1, L.F64, D.Slab(), (1, 2, 3))
+    pi = D.Plan(shape, L.C2C, +1, L.F64, D.Slab(), (1, 2, 3), flags=L.FLAG_PLANE_PIPELINE)
     assert pi.schedule()[0] == "plane_pipeline"
     B = pi.execute(Fs, Fs.similar())
     assert O.rel_l2(B.to_host(), a) <= TOL64
-    # f32 through the same persistent kernel
+    # f32 through the same grouped kernel
     a32 = O.fill(18, shape, np.complex64)
-    F32 = D.fft(_dev(a32), decomp=D.Slab())
+    p32 = D.Plan(shape, L.C2C, -1, L.F32, D.Slab(), (1, 2, 3), flags=L.FLAG_PLANE_PIPELINE)
+    F32 = p32.execute(_dev(a32), D.DeviceArray(shape, np.complex64))
     assert O.rel_l2(F32.to_host().astype(np.complex128), O.fftn(a32.astype(np.complex128))) <= TOL32
