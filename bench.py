@@ -39,6 +39,9 @@ def parse():
     ap.add_argument("--decomp", choices=["pencil", "slab", "both"], default="both")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--profile-steps", action="store_true", help="no per-pass events (for rocprof runs)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
+    ap.add_argument("--serve-only", action="store_true", help="only the served-stream measurement")
+    ap.add_argument("--serve-count", type=int, default=2000, help="requests in the served trace")
     return ap.parse_args()
 
 
@@ -79,6 +82,81 @@ def cpu_baseline(n):
             "(per-axis batched FFTs in the reference's stage order), time per transform %.3f s" % (n, t)}
 
 
+def serving(count):
+    """The served mixed-size stream: a Poisson trace (200 req/s, seed 8) of
+    60% C2C / 40% R2C cubic transforms, edges log-uniform in [64, 512] snapped
+    to even 7-smooth sizes, ComplexF32/ComplexF64 50/50, inputs generated on
+    device from seed 10000 + r, under a 200 GB HBM budget.
+
+      open_loop     requests submitted at their arrival times: p50/p99
+                    latency from arrival to completion (plans are built at the
+                    first request of each shape, inside the run);
+      saturating    the same requests submitted at once: throughput;
+      sequential    the same requests as plain one-at-a-time calls (rand,
+                    fft/rfft, synchronize): warm = plan cache and caching
+                    allocator kept (plans pre-built), cold = every call builds
+                    its plan and allocates its arrays and frees both after.
+    Served results are checked against the warm sequential results on a
+    sample of requests."""
+    import numpy as np
+    import torch
+
+    from daggerfft import serve as S
+    from daggerfft import _lib as L
+    import daggerfft as D
+
+    trace = S.serving_trace(count=count)
+    keep = [r for r in range(0, count, 25) if trace[r].shape[0] <= 256]
+    out = {"trace": "%d requests, Poisson 200 req/s (seed 8), 60%% C2C / 40%% R2C, edge^3 with edge "
+                    "log-uniform in [64,512] snapped to even 7-smooth sizes, CF32/CF64 50/50, "
+                    "inputs from seeds 10000+r generated on device" % count,
+           "hbm_budget_bytes": 200 * 10 ** 9, "lanes": 4, "max_inflight": 32,
+           "distinct_shapes": len(set((q.shape, q.kind, q.precision) for q in trace)),
+           "algorithmic_bytes_total": float(sum(q.algorithmic_bytes() for q in trace))}
+    with S.Server(hbm_budget=200 * 10 ** 9, max_inflight=32, lanes=4) as srv:
+        ol = S.replay(srv, trace, open_loop=True, keep=keep)
+        sat = S.replay(srv, trace, open_loop=False)
+        st = srv.stats()
+    served = ol.pop("results")
+    sat.pop("results")
+    for d in (ol, sat):
+        d.pop("latencies")
+    out["open_loop"] = {k: round(v, 4) if isinstance(v, float) else v for k, v in ol.items()}
+    out["saturating"] = {k: round(v, 4) if isinstance(v, float) else v for k, v in sat.items()
+                         if not k.startswith("latency")}
+    out["server_stats"] = {k: st[k] for k in ("plans", "peak_running", "overtakes", "buffer_allocs",
+                                              "buffer_reuses", "buffer_evictions", "peak_bytes_held")}
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    # warm sequential: plan cache pre-built for every shape, as the server's plans are
+    for q in {(q.shape, q.kind, q.precision): q for q in trace}.values():
+        x = D.rand(q.shape, q.in_dtype, seed=1)
+        (D.rfft(x) if q.kind == L.R2C else D.fft(x))
+    torch.cuda.synchronize()
+    warm = S.sequential_plain(trace, keep=keep)
+    ref = warm.pop("results")
+    cold = S.sequential_plain(trace, cold=True)
+    cold.pop("results")
+    out["sequential_warm"] = {k: round(v, 4) for k, v in warm.items()}
+    out["sequential_cold"] = {k: round(v, 4) for k, v in cold.items()}
+    out["throughput_vs_sequential_warm"] = round(sat["req_per_s"] / warm["req_per_s"], 3)
+    out["throughput_vs_sequential_cold"] = round(sat["req_per_s"] / cold["req_per_s"], 3)
+    worst = {"f32": 0.0, "f64": 0.0}
+    same = 0
+    for r in keep:
+        a = ref[r].tensor
+        b = served[r].tensor
+        same += int(torch.equal(a, b))
+        e = float(torch.linalg.vector_norm(b - a) / torch.linalg.vector_norm(a))
+        k = "f64" if trace[r].precision == L.F64 else "f32"
+        worst[k] = max(worst[k], e)
+    out["served_vs_sequential"] = {"sampled": len(keep), "bit_identical": same, "max_rel_l2": worst,
+                                   "tolerance": {"f64": 1e-12, "f32": 1e-6}}
+    D.clear_plan_cache()
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -92,6 +170,9 @@ def main():
     from daggerfft import _lib as L
 
     torch.cuda.set_device(0)
+    if args.serve_only:
+        print(json.dumps({"serving": serving(args.serve_count)}), flush=True)
+        return
     n = args.n
     shape = (n, n, n)
     N = float(n) ** 3
@@ -237,7 +318,13 @@ def main():
         "roofline": roofline,
         "cpu_baseline": None,
     }
-    del X, Y
+    del X, Y, A
+    torch.cuda.empty_cache()
+    if not args.no_serve:
+        try:
+            out["serving"] = serving(args.serve_count)
+        except Exception as e:  # reported, never fatal
+            out["serving"] = {"error": repr(e)}
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(n)

@@ -528,6 +528,8 @@ PassPlan plan_pass(const dfft_plan *p, int i) { return p->passes[i]; }
 size_t plan_buffer_bytes(const dfft_plan *p, int buf) {
     return buf == BUF_IN ? p->in_bytes : buf == BUF_OUT ? p->out_bytes : p->work_bytes;
 }
+// device memory the plan itself holds (twiddle tables, its own workspace if allocated)
+size_t plan_resident_bytes(const dfft_plan *p) { return p->tw_bytes + (p->own_work ? p->work_bytes : 0); }
 }  // namespace dfft
 
 static void free_plan(dfft_plan *p) {

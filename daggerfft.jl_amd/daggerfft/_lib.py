@@ -52,6 +52,35 @@ class dfft_batch_desc(ctypes.Structure):
                 ("precision", ctypes.c_int), ("flags", ctypes.c_uint32)]
 
 
+class dfft_server_config(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_inflight", ctypes.c_int), ("lanes", ctypes.c_int),
+                ("flags", ctypes.c_uint32), ("reserved", ctypes.c_int)]
+
+
+class dfft_request(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64 * 3), ("kind", ctypes.c_int), ("sign", ctypes.c_int),
+                ("precision", ctypes.c_int), ("decomp", ctypes.c_int), ("flags", ctypes.c_uint32),
+                ("req_flags", ctypes.c_uint32), ("seed", ctypes.c_uint64), ("d_in", ctypes.c_void_p),
+                ("d_out", ctypes.c_void_p), ("t_arrival", ctypes.c_double)]
+
+
+class dfft_request_info(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int), ("status", ctypes.c_int), ("lane", ctypes.c_int),
+                ("reserved", ctypes.c_int), ("t_arrival", ctypes.c_double), ("t_admit", ctypes.c_double),
+                ("t_done", ctypes.c_double), ("d_result", ctypes.c_void_p), ("result_bytes", ctypes.c_size_t),
+                ("algorithmic_bytes", ctypes.c_double)]
+
+
+class dfft_server_stats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int64) for k in ("submitted", "completed", "failed", "queued", "running",
+                                              "peak_running", "plans", "overtakes", "buffer_allocs",
+                                              "buffer_reuses", "buffer_evictions")] + \
+               [(k, ctypes.c_size_t) for k in ("budget", "bytes_held", "bytes_busy", "peak_bytes_held")]
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
+
+
 def _declare(L):
     P = ctypes.c_void_p
     I = ctypes.c_int
@@ -87,6 +116,18 @@ def _declare(L):
         "dfft_tune_entries": [IP],
         "dfft_batch_algorithmic_bytes": [P, ctypes.POINTER(ctypes.c_double)],
         "dfft_batch_num_launches": [P, IP],
+        "dfft_server_create": [ctypes.POINTER(P), ctypes.POINTER(dfft_server_config)],
+        "dfft_server_clock": [P, ctypes.POINTER(ctypes.c_double)],
+        "dfft_server_prepare": [P, ctypes.POINTER(dfft_request), I],
+        "dfft_server_submit": [P, ctypes.POINTER(dfft_request), I64P],
+        "dfft_server_wait": [P, ctypes.c_int64, ctypes.c_double],
+        "dfft_server_wait_any": [P, ctypes.c_double, I64P],
+        "dfft_server_info": [P, ctypes.c_int64, ctypes.POINTER(dfft_request_info)],
+        "dfft_server_copy_result": [P, ctypes.c_int64, P, ctypes.c_size_t, P],
+        "dfft_server_release": [P, ctypes.c_int64],
+        "dfft_server_drain": [P, ctypes.c_double],
+        "dfft_server_stats_get": [P, ctypes.POINTER(dfft_server_stats)],
+        "dfft_server_destroy": [P],
     }
     for name, args in sig.items():
         f = getattr(L, name, None)

@@ -29,8 +29,9 @@
  *                        (src/fftgpu.jl:452-495) and pack_data!/unpack_data!/
  *                        local_copy! (src/fft.jl:180-231)
  *   dfft_batch_*         no reference counterpart: many independent transforms
- *                        in one persistent launch
- *   dfft_server_*        no reference counterpart: continuous-batching request loop
+ *                        per launch (one launch per axis-pass level)
+ *   dfft_server_*        no reference counterpart: continuous-batching request
+ *                        loop with resident plans and pooled buffers
  */
 #ifndef DAGGERFFT_HIP_H
 #define DAGGERFFT_HIP_H
@@ -227,6 +228,110 @@ int dfft_batch_num_items(const dfft_batch *batch, int64_t *items);
 int dfft_batch_num_launches(const dfft_batch *batch, int *launches);
 int dfft_batch_algorithmic_bytes(const dfft_batch *batch, double *bytes);
 int dfft_batch_destroy(dfft_batch *batch);
+
+/* ---- serving loop ----------------------------------------------------------
+ * No reference counterpart as a whole.  It serves a stream of independent 3D
+ * transform requests with what the reference rebuilds per call kept
+ * resident: the plan cache (GPU_PLAN_CACHE / get_or_create_gpu_plan, src/fftgpu.jl:83,107-116)
+ * and the device staging buffers of a workspace (create_gpu_workspace,
+ * src/fftgpu.jl:227-249, released by gpu_cleanup_workspace!,
+ * src/fftgpu.jl:630-634).
+ *
+ * Clients submit requests from any thread; submission never blocks on the
+ * device.  A scheduler thread admits queued requests in arrival order as
+ * in-flight slots and memory under the HBM budget free up (a later request
+ * that fits may overtake one waiting for memory; the oldest waiting request
+ * keeps its memory reserved, so it is never starved), launches each on the
+ * least-loaded of `lanes` HIP streams -- work of requests that arrive while
+ * others run overlaps on the device -- and retires it when its completion
+ * event fires.  Plans are built once per (shape, kind, sign, precision,
+ * decomp, flags), at the first submission of that shape, and kept until
+ * dfft_server_destroy.  Device buffers are pooled by size class and reused;
+ * idle buffers are freed only when the budget needs the room for another
+ * class.  A request's result is the same as dfft_execute of the same plan on
+ * the same input (the same kernels run). */
+typedef struct dfft_server dfft_server;
+
+typedef struct {
+    size_t hbm_budget;      /* device bytes the server may hold: pooled buffers + plans (0: 90% of free memory) */
+    int max_inflight;       /* requests admitted and not yet retired (0: 32) */
+    int lanes;              /* HIP streams the requests are spread over (0: 4) */
+    uint32_t flags;         /* reserved, must be 0 */
+    int reserved;
+} dfft_server_config;
+
+typedef struct {
+    int64_t n[3];           /* 3D size, n[0] fastest; transforms run over dims (0,1,2) */
+    int kind;               /* DFFT_C2C, DFFT_R2C or DFFT_C2R */
+    int sign;               /* -1 / +1 for C2C; -1 for R2C; +1 for C2R */
+    int precision;          /* DFFT_F32 / DFFT_F64 */
+    int decomp;             /* DFFT_PENCIL / DFFT_SLAB */
+    uint32_t flags;         /* plan flags (DFFT_FLAG_UNNORMALIZED, DFFT_FLAG_NO_TUNE, ...) */
+    uint32_t req_flags;     /* reserved, must be 0 */
+    uint64_t seed;          /* d_in == NULL: the input is dfft_fill_uniform(seed) of the input shape */
+    const void *d_in;       /* the caller's device input, read and left untouched; or NULL */
+    void *d_out;            /* the caller's device output; or NULL: the result stays in a server
+                               buffer until dfft_server_release */
+    double t_arrival;       /* arrival time on the server clock (dfft_server_clock); 0: time of submission */
+} dfft_request;
+
+#define DFFT_REQ_QUEUED   0
+#define DFFT_REQ_RUNNING  1
+#define DFFT_REQ_DONE     2
+#define DFFT_REQ_FAILED   3
+
+typedef struct {
+    int state;              /* DFFT_REQ_* */
+    int status;             /* DFFT_OK once done, the error once failed, DFFT_ERR_NOT_READY before */
+    int lane;               /* stream it ran on (-1 before admission) */
+    int reserved;
+    double t_arrival;       /* server clock, seconds */
+    double t_admit;
+    double t_done;          /* when the scheduler saw its completion event */
+    const void *d_result;   /* once done: the caller's d_out or the server buffer holding the result */
+    size_t result_bytes;
+    double algorithmic_bytes;
+} dfft_request_info;
+
+typedef struct {
+    int64_t submitted, completed, failed;
+    int64_t queued, running, peak_running;
+    int64_t plans;          /* resident plans */
+    int64_t overtakes;      /* admissions that overtook a request waiting for memory */
+    int64_t buffer_allocs, buffer_reuses, buffer_evictions;
+    size_t budget;
+    size_t bytes_held;      /* pooled buffers (busy + idle) + plans */
+    size_t bytes_busy;      /* buffers of running requests and of unreleased results */
+    size_t peak_bytes_held;
+} dfft_server_stats;
+
+int dfft_server_create(dfft_server **server, const dfft_server_config *config);
+/* Seconds on the server's clock (monotonic, 0 at creation). */
+int dfft_server_clock(const dfft_server *server, double *now);
+/* Build and keep the plans of `count` request shapes ahead of their first submission. */
+int dfft_server_prepare(dfft_server *server, const dfft_request *requests, int count);
+/* Queue a request; *ticket identifies it.  DFFT_ERR_BUDGET if it can never fit
+ * the budget; plan errors (DFFT_ERR_UNSUPPORTED_SIZE, ...) are returned here.
+ * The caller's d_in must hold its data when submitted (the server's streams do
+ * not wait for the caller's), and d_in/d_out must stay valid until the
+ * request is done. */
+int dfft_server_submit(dfft_server *server, const dfft_request *request, int64_t *ticket);
+/* Wait up to timeout_s (< 0: forever) for one request: DFFT_OK when it is
+ * done or failed (see dfft_server_info), DFFT_ERR_NOT_READY on time-out. */
+int dfft_server_wait(dfft_server *server, int64_t ticket, double timeout_s);
+/* Next finished request not yet returned by this call, in completion order. */
+int dfft_server_wait_any(dfft_server *server, double timeout_s, int64_t *ticket);
+int dfft_server_info(dfft_server *server, int64_t ticket, dfft_request_info *info);
+/* Copy a finished request's result (result_bytes) to d_dst, ordered on `stream`. */
+int dfft_server_copy_result(dfft_server *server, int64_t ticket, void *d_dst, size_t bytes, void *stream);
+/* Return a finished request's buffers to the pool and forget its ticket;
+ * a queued request is cancelled.  DFFT_ERR_STATE while it runs. */
+int dfft_server_release(dfft_server *server, int64_t ticket);
+/* Wait until nothing is queued or running. */
+int dfft_server_drain(dfft_server *server, double timeout_s);
+int dfft_server_stats_get(dfft_server *server, dfft_server_stats *stats);
+/* Waits for running requests, then frees everything the server holds. */
+int dfft_server_destroy(dfft_server *server);
 
 #ifdef __cplusplus
 }
