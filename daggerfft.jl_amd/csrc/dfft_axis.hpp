@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
+#include <utility>
 
 #include "dfft_math.hpp"
 #include "dfft_types.hpp"
@@ -341,6 +342,45 @@ struct Radices {
         return p;
     }
 };
+
+// Compile-time factorization in the runtime planner's greedy order: radix-8
+// stages first, then 4, 2, 3, 5, 7.
+struct Factors {
+    int S;
+    int R[MAX_STAGES];
+};
+constexpr Factors factors_of(int L) {
+    Factors f{0, {0}};
+    constexpr int order[6] = {8, 4, 2, 3, 5, 7};
+    int n = L;
+    for (int i = 0; i < 6; ++i)
+        while (n > 1 && n % order[i] == 0 && f.S < MAX_STAGES) {
+            f.R[f.S++] = order[i];
+            n /= order[i];
+        }
+    if (n != 1) f.S = -1;
+    return f;
+}
+template <int L, class Seq>
+struct RadicesOfImpl;
+template <int L, int... I>
+struct RadicesOfImpl<L, std::integer_sequence<int, I...>> {
+    using type = Radices<factors_of(L).R[I]...>;
+};
+template <int L>
+using RadicesOf = typename RadicesOfImpl<L, std::make_integer_sequence<int, factors_of(L).S>>::type;
+
+// threads per line such that each stage's L/R butterflies fit EMAX/R per thread
+template <class RL, int EMAX>
+constexpr int fixed_tpl() {
+    int t = 1;
+    for (int s = 0; s < RL::S; ++s) {
+        const int nb = RL::L / RL::R[s], per = EMAX / RL::R[s];
+        const int need = (nb + per - 1) / per;
+        if (need > t) t = need;
+    }
+    return t;
+}
 
 template <class RL, int EMAX, int TPLC, bool SPLIT, int s, class T, int OP, bool CONTIG>
 __device__ __forceinline__ void fixed_stages_from(const TileCtx<T, OP, CONTIG> &c, cx<T> *v) {

@@ -1,0 +1,67 @@
+// dfft_fixed.hpp -- the catalog of compile-time axis-pass kernels.
+//
+// A catalog kernel has its length, radices, values per thread and threads
+// per line fixed at compile time: every stage is unrolled with constant
+// radix and constant index arithmetic (j mod Ns is a mask or a multiply),
+// which keeps the register count low enough for several tiles per CU.
+//
+// Variants (AxisPass::variant):
+//   1  8 values per thread, whole complex values exchanged through LDS
+//   2  16 values per thread, real and imaginary parts exchanged in two
+//      rounds through half the LDS (strided FP64 lines of >= 256 points)
+//   3  16 values per thread, whole complex values exchanged through LDS
+//      (the mixed-radix lengths: radix-3/5/7 stages leave at most 1-2 of
+//      the 16 registers idle, against 3 of 8)
+// The catalog is spread over several translation units so it compiles in
+// parallel; fixed_entry() in dfft_kernels.hip searches all of them.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "dfft_axis.hpp"
+
+namespace dfft {
+
+using axis_kernel_t = void (*)(AxisPass);
+
+template <class T, int OP, bool CONTIG, int EMAX, bool SPLIT, class RL, int TPLC>
+__global__ __launch_bounds__(1024) void k_axis_fixed(AxisPass a) {
+    extern __shared__ __align__(16) unsigned char dfft_smem[];
+    axis_tile<EMAX, T, OP, CONTIG, SPLIT, RL, TPLC>(a, a.in, a.out, (int64_t)blockIdx.x, (int)threadIdx.x,
+                                                     reinterpret_cast<cx<T> *>(dfft_smem));
+}
+
+struct FixedEntry {
+    int prec, op, contig, variant, L, tpl, S;
+    const int *R;
+    axis_kernel_t fn;
+};
+
+struct FixedTable {
+    const FixedEntry *e;
+    int n;
+};
+
+#define DFFT_FX(T, P, OPV, CONTIG, VAR, EMAXV, SPLIT, RL)                                             \
+    {P, OPV, CONTIG, VAR, RL::L, fixed_tpl<RL, EMAXV>(), RL::S, RL::R,                                 \
+     (axis_kernel_t)k_axis_fixed<T, OPV, CONTIG, EMAXV, SPLIT, RL, fixed_tpl<RL, EMAXV>()>}
+
+// FP64 C2C of length L: contiguous (3), strided split (2), strided (3)
+#define DFFT_C2C_F64(L)                                                \
+    DFFT_FX(double, 1, OP_C2C, 1, 3, 16, false, RadicesOf<L>),        \
+        DFFT_FX(double, 1, OP_C2C, 0, 2, 16, true, RadicesOf<L>),     \
+        DFFT_FX(double, 1, OP_C2C, 0, 3, 16, false, RadicesOf<L>)
+// FP32 C2C of length L: contiguous (3), strided (3)
+#define DFFT_C2C_F32(L)                                                \
+    DFFT_FX(float, 0, OP_C2C, 1, 3, 16, false, RadicesOf<L>),         \
+        DFFT_FX(float, 0, OP_C2C, 0, 3, 16, false, RadicesOf<L>)
+// R2C of a real line of 2H points through an H-point FFT (contiguous axis)
+#define DFFT_R2C(H)                                                    \
+    DFFT_FX(double, 1, OP_R2C, 1, 3, 16, false, RadicesOf<H>),        \
+        DFFT_FX(float, 0, OP_R2C, 1, 3, 16, false, RadicesOf<H>)
+
+#define DFFT_TABLE(name, ...)                                           \
+    static const FixedEntry name##_e[] = {__VA_ARGS__};                 \
+    extern const FixedTable name;                                       \
+    const FixedTable name = {name##_e, (int)(sizeof(name##_e) / sizeof(name##_e[0]))};
+
+}  // namespace dfft

@@ -1,0 +1,20 @@
+// dfft_fixed_pow2.hip -- C2C catalog kernels of the power-of-two lengths
+// 64..1024: 8 values per thread (variant 1, both layouts) and the split-LDS
+// strided schedule (variant 2).
+#include "dfft_fixed.hpp"
+
+namespace dfft {
+
+#define DFFT_POW2(T, P, L)                                              \
+    DFFT_FX(T, P, OP_C2C, 1, 1, 8, false, RadicesOf<L>),               \
+        DFFT_FX(T, P, OP_C2C, 0, 1, 8, false, RadicesOf<L>),           \
+        DFFT_FX(T, P, OP_C2C, 0, 2, 16, true, RadicesOf<L>)
+
+DFFT_TABLE(g_fixed_pow2,
+    DFFT_POW2(double, 1, 64), DFFT_POW2(double, 1, 128), DFFT_POW2(double, 1, 256),
+    DFFT_POW2(double, 1, 512), DFFT_POW2(double, 1, 1024),
+    DFFT_POW2(float, 0, 64), DFFT_POW2(float, 0, 128), DFFT_POW2(float, 0, 256),
+    DFFT_POW2(float, 0, 512), DFFT_POW2(float, 0, 1024)
+)
+
+}  // namespace dfft

@@ -300,7 +300,7 @@ int dfft_batch_create(dfft_batch **batch, int count, const dfft_batch_desc *desc
             Tuning tn;
             const Tuning *tune = nullptr;
             if (!(flags & DFFT_FLAG_NO_TUNE) && tuning_lookup(pp, tn) && tn.tpl <= 512) tune = &tn;
-            rc = schedule_pass(pp, tune, 512, 65536, 256, false);
+            rc = schedule_pass(pp, tune, 512, 65536, 256, true);
             if (rc != DFFT_OK) break;
             (prec ? tt64 : tt32).add(pp.ap);
             lv[i].push_back(pp);

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include "dfft_axis.hpp"
+#include "dfft_fixed.hpp"
 #include "dfft_kernels.hpp"
 #include "dfft_plan.hpp"
 
@@ -25,8 +26,6 @@ __global__ __launch_bounds__(1024) void k_axis(AxisPass a) {
                                    reinterpret_cast<cx<T> *>(dfft_smem));
 }
 
-using axis_kernel_t = void (*)(AxisPass);
-
 #define DFFT_OPS(T, CONTIG)                                                                   \
     {(axis_kernel_t)k_axis<T, OP_C2C, CONTIG, 8>, (axis_kernel_t)k_axis<T, OP_R2C, CONTIG, 8>, \
      (axis_kernel_t)k_axis<T, OP_R2C_FULL, CONTIG, 8>, (axis_kernel_t)k_axis<T, OP_C2R, CONTIG, 8>, \
@@ -40,45 +39,22 @@ static axis_kernel_t g_axis_kernels[2][2][OP_COUNT] = {
 };
 
 // ---------------------------------------------------------------------------
-// compile-time schedules of the hot C2C lengths: every stage unrolled with
-// constant radix, length and threads per line
+// compile-time schedules (the catalog tables live in dfft_fixed_*.hip)
 // ---------------------------------------------------------------------------
-template <class T, bool CONTIG, int EMAX, bool SPLIT, class RL, int TPLC>
-__global__ __launch_bounds__(1024) void k_axis_fixed(AxisPass a) {
-    extern __shared__ __align__(16) unsigned char dfft_smem[];
-    axis_tile<EMAX, T, OP_C2C, CONTIG, SPLIT, RL, TPLC>(a, a.in, a.out, (int64_t)blockIdx.x, (int)threadIdx.x,
-                                                        reinterpret_cast<cx<T> *>(dfft_smem));
-}
+extern const FixedTable g_fixed_pow2, g_fixed_f64a, g_fixed_f64b, g_fixed_f32a, g_fixed_f32b, g_fixed_r2ca,
+    g_fixed_r2cb;
+static const FixedTable *const g_fixed_tables[] = {&g_fixed_pow2, &g_fixed_f64a, &g_fixed_f64b, &g_fixed_f32a,
+                                                   &g_fixed_f32b, &g_fixed_r2ca, &g_fixed_r2cb};
 
-struct FixedEntry {
-    int prec, contig, variant, L, tpl, S;
-    int R[MAX_STAGES];
-    axis_kernel_t fn;
-};
-
-#define DFFT_FIXED(T, P, RL, ...)                                                                          \
-    {P, 1, 1, RL::L, RL::L / 8, RL::S, {__VA_ARGS__}, (axis_kernel_t)k_axis_fixed<T, true, 8, false, RL, RL::L / 8>},  \
-    {P, 0, 1, RL::L, RL::L / 8, RL::S, {__VA_ARGS__}, (axis_kernel_t)k_axis_fixed<T, false, 8, false, RL, RL::L / 8>}, \
-    {P, 0, 2, RL::L, RL::L / 16, RL::S, {__VA_ARGS__}, (axis_kernel_t)k_axis_fixed<T, false, 16, true, RL, RL::L / 16>}
-
-using R64 = Radices<8, 8>;
-using R128 = Radices<8, 8, 2>;
-using R256 = Radices<8, 8, 4>;
-using R512 = Radices<8, 8, 8>;
-using R1024 = Radices<8, 8, 8, 2>;
-
-static const FixedEntry g_fixed[] = {
-    DFFT_FIXED(double, 1, R64, 8, 8),        DFFT_FIXED(double, 1, R128, 8, 8, 2),
-    DFFT_FIXED(double, 1, R256, 8, 8, 4),    DFFT_FIXED(double, 1, R512, 8, 8, 8),
-    DFFT_FIXED(double, 1, R1024, 8, 8, 8, 2), DFFT_FIXED(float, 0, R64, 8, 8),
-    DFFT_FIXED(float, 0, R128, 8, 8, 2),     DFFT_FIXED(float, 0, R256, 8, 8, 4),
-    DFFT_FIXED(float, 0, R512, 8, 8, 8),     DFFT_FIXED(float, 0, R1024, 8, 8, 8, 2),
-};
-
-const FixedEntry *fixed_entry(int prec, int contig, int variant, int op, int L) {
-    if (op != OP_C2C || variant < 1) return nullptr;
-    for (const auto &e : g_fixed)
-        if (e.prec == (prec ? 1 : 0) && e.contig == (contig ? 1 : 0) && e.variant == variant && e.L == L) return &e;
+static const FixedEntry *fixed_entry(int prec, int contig, int variant, int op, int L) {
+    if (variant < 1) return nullptr;
+    for (const FixedTable *t : g_fixed_tables)
+        for (int i = 0; i < t->n; ++i) {
+            const FixedEntry &e = t->e[i];
+            if (e.prec == (prec ? 1 : 0) && e.op == op && e.contig == (contig ? 1 : 0) && e.variant == variant &&
+                e.L == L)
+                return &e;
+        }
     return nullptr;
 }
 

@@ -161,34 +161,39 @@ static bool tpl_ok(int L, const std::vector<int> &R, int emax, int tpl) {
 // schedule of one pass: radices + launch shape
 // ---------------------------------------------------------------------------
 int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_lds, int contig_threads,
-                  bool allow_split) {
+                  bool grouped) {
     AxisPass &a = pp.ap;
     const int L = a.sp.L;
     const int es = pp.prec ? 16 : 8;
-    // variant 1/2: a compile-time schedule of this length (2 also halves the
-    // LDS by exchanging real and imaginary parts in two rounds; strided only)
-    // default: the split compile-time schedule for strided FP64 lines of >= 256
-    // points (two tiles per CU fit in LDS), else the compile-time schedule
-    int want = tune ? tune->variant : ((!a.contig && pp.prec && L >= 256) ? 2 : 1);
-    if (want == 2 && (!allow_split || a.contig)) want = 1;
+    // variants 1-3: a compile-time schedule of this length (see dfft_fixed.hpp).
+    // Default: the split schedule for strided FP64 lines of >= 256 points (two
+    // tiles per CU fit in LDS), else 8 values per thread where the catalog
+    // has it (powers of two), else 16.
+    std::vector<int> prefs;
+    if (tune) prefs = {tune->variant};
+    else if (!a.contig && pp.prec && L >= 256) prefs = {2, 1, 3};
+    else prefs = {1, 3};
+    if (tune && tune->variant >= 2) prefs.push_back(1);
     std::vector<int> R;
     int fixed_tpl = 0;
     a.variant = 0;
-    if (want >= 1 && fixed_schedule(pp.prec, a.contig, want, a.op, L, R, fixed_tpl)) {
-        a.variant = want;
-    } else if (want == 2 && fixed_schedule(pp.prec, a.contig, 1, a.op, L, R, fixed_tpl)) {
-        a.variant = 1;
-    } else if (tune && !tune->radices.empty()) {
-        R = tune->radices;
-    } else if (!factorize(L, R)) {
-        return DFFT_ERR_UNSUPPORTED_SIZE;
+    for (int want : prefs) {
+        if (want < 1 || (want == 2 && a.contig) || (grouped && want != 1)) continue;
+        if (fixed_schedule(pp.prec, a.contig, want, a.op, L, R, fixed_tpl)) {
+            a.variant = want;
+            break;
+        }
+    }
+    if (!a.variant) {
+        if (tune && !tune->radices.empty()) R = tune->radices;
+        else if (!factorize(L, R)) return DFFT_ERR_UNSUPPORTED_SIZE;
     }
     int prod = 1;
     for (int r : R) prod *= r;
     if (prod != L || (int)R.size() > MAX_STAGES) return DFFT_ERR_UNSUPPORTED_SIZE;
     a.sp.S = (int)R.size();
     for (int s = 0; s < a.sp.S; ++s) a.sp.R[s] = R[s];
-    const int emax = a.variant == 2 ? 16 : EMAX;
+    const int emax = a.variant >= 2 ? 16 : EMAX;
     const int lds_es = a.variant == 2 ? es / 2 : es;
     int tpl = a.variant ? fixed_tpl
                         : (tune && tune->tpl > 0 && tpl_ok(L, R, emax, tune->tpl)) ? tune->tpl
@@ -215,7 +220,9 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
         a.pad_shift = 30;
         // 256-byte (FP64) / 512-byte rows; the split schedule doubles the row over outer slices
         int tc = (tune && tune->tc > 0) ? tune->tc
-                                        : (a.variant == 2 ? (a.nouter > 1 ? 32 : 16) : (pp.prec ? 16 : 32));
+                 : a.variant == 2         ? (a.nouter > 1 ? 32 : 16)
+                 : a.variant == 3         ? 16
+                                          : (pp.prec ? 16 : 32);
         int64_t cols = a.nlines;
         while (tc > 1 && tc / 2 >= cols) tc /= 2;
         while (tc > 1 && (int64_t)tc * tpl > max_threads) tc /= 2;
@@ -451,9 +458,19 @@ int build_plan(dfft_plan *p) {
         int64_t k0, planes;
     };
     std::vector<ChunkPass> cpa, cpb;  // chunk passes of the first / second 2D-stage pass
+    PassPlan line;                    // the dims[2] pass, scheduled for the grouped kernel
     if (slab_ok) {
         const bool inv = p->sign > 0;
         const int ia = inv ? 1 : 0, ib = inv ? 2 : 1;
+        line = p->passes[inv ? 0 : 2];
+        {
+            const Tuning *tune = nullptr;
+            Tuning t;
+            if (!(p->flags & DFFT_FLAG_NO_TUNE) && tuning_lookup(line, t)) tune = &t;
+            int rc2 = schedule_pass(line, tune, 1024, 0, 256, true);
+            if (rc2 != DFFT_OK) return rc2;
+            tt.add(line.ap);
+        }
         auto sizes = [&](const PassPlan &q, int64_t *nin, int64_t *nout) {
             const bool in_real = q.ap.op == OP_R2C || q.ap.op == OP_R2C_FULL;
             const bool out_real = q.ap.op == OP_C2R || q.ap.op == OP_C2R_FULL;
@@ -478,7 +495,7 @@ int build_plan(dfft_plan *p) {
                 const Tuning *tune = nullptr;
                 Tuning t;
                 if (!(p->flags & DFFT_FLAG_NO_TUNE) && tuning_lookup(q, t)) tune = &t;
-                int rc2 = schedule_pass(q, tune, 1024, 0, 1024, false);
+                int rc2 = schedule_pass(q, tune, 1024, 0, 1024, true);
                 if (rc2 != DFFT_OK) return rc2;
                 q.alg_bytes = full.alg_bytes * (double)planes / (double)n[2];
                 q.ap.nt = 0;  // the next launch re-reads this chunk from the Infinity Cache
@@ -504,7 +521,7 @@ int build_plan(dfft_plan *p) {
                                    c.k0 * plane_bytes(c.pp, false));
         };
         std::vector<std::vector<GroupItem>> levels;
-        const PassPlan &line = inv ? p->passes[0] : p->passes[2];
+        line.ap.tw = p->d_tw;
         if (inv) levels.push_back({make_group_item(line, line.src, 0, line.dst, 0)});
         const size_t C = cpa.size();
         for (size_t i = 0; i <= C; ++i) {
@@ -751,8 +768,9 @@ int dfft_plan_describe(const dfft_plan *p, int pass, char *buf, size_t buflen) {
     const char *T = pp.prec ? "double" : "float";
     std::string kname;
     if (a.variant >= 1)
-        kname = std::string("k_axis_fixed<") + T + "," + (a.contig ? "true" : "false") + "," +
-                (a.variant == 2 ? "16,true" : "8,false") + ",dfft::Radices<" + rl + ">," + std::to_string(a.TPL) + ">";
+        kname = std::string("k_axis_fixed<") + T + "," + std::to_string(a.op) + "," + (a.contig ? "true" : "false") +
+                "," + (a.variant == 2 ? "16,true" : a.variant == 3 ? "16,false" : "8,false") + ",dfft::Radices<" + rl +
+                ">," + std::to_string(a.TPL) + ">";
     else
         kname = std::string("k_axis<") + T + "," + std::to_string(a.op) + "," + (a.contig ? "true" : "false") + ",8>";
     snprintf(buf, buflen,

@@ -41,8 +41,11 @@ struct Tuning {
 int hip_fail(int hip_status, const char *what);
 bool factorize(int L, std::vector<int> &R);
 void set_geometry(AxisPass &a, int axis, const int64_t nin[3], const int64_t nout[3]);
+// grouped: the pass runs in a grouped launch (dfft_group.hip), whose kernel
+// follows the runtime schedule with 8 values per thread -- only schedules
+// with that shape (variants 0 and 1) qualify.
 int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads = 1024, size_t max_lds = 0,
-                  int contig_threads = 256, bool allow_split = true);
+                  int contig_threads = 256, bool grouped = false);
 
 // Stage twiddle + pre/post blocks of every distinct (op, L, radices) of one
 // precision, in one device table.

@@ -137,7 +137,7 @@ static std::vector<Tuning> candidates(const PassPlan &pp, int level, int max_thr
     std::vector<int> pads = a.contig ? (level >= 2 ? std::vector<int>{3, 4, 5} : std::vector<int>{3, 4})
                                      : std::vector<int>{0};
     // compile-time schedules of this length: only the tile shape varies
-    for (int var : {1, 2}) {
+    for (int var : {1, 2, 3}) {
         std::vector<int> R;
         int tpl = 0;
         if (!fixed_schedule(pp.prec, a.contig, var, a.op, a.sp.L, R, tpl)) continue;
@@ -316,7 +316,7 @@ static int load_file(const char *path) {
         Tuning t;
         if (!(ss >> key >> rad >> t.tpl >> t.tc >> t.pad_shift)) return DFFT_ERR_ARG;
         if (!(ss >> t.variant)) t.variant = 0;
-        if (t.variant < 0 || t.variant > 2) return DFFT_ERR_ARG;
+        if (t.variant < 0 || t.variant > 3) return DFFT_ERR_ARG;
         if (rad != "-") {
             std::stringstream rs(rad);
             std::string tok;

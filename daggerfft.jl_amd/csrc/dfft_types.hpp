@@ -48,7 +48,8 @@ struct AxisPass {
     int nt;                   // 1: streaming (nontemporal) global loads/stores
     int variant;              // 0: runtime schedule (any length); 1: compile-time schedule of this
                               // length, 8 values/thread; 2: compile-time schedule, 16 values/thread,
-                              // real and imaginary parts exchanged in two rounds (half the LDS)
+                              // real and imaginary parts exchanged in two rounds (half the LDS);
+                              // 3: compile-time schedule, 16 values/thread
     int64_t nlines;           // contig: number of lines; strided: number of columns
     int64_t nouter;           // strided: number of outer slices (1 for contig)
     int64_t in_sm, in_so;     // input: stride along the line, stride between lines/outer

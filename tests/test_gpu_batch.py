@@ -1,6 +1,8 @@
 """Batched launch of 64 mixed-size 3D transforms: every result equals the
-same transform planned and run alone, and the oracle within tolerance
-(ComplexF64 rel-L2 <= 1e-12; ComplexF32 vs the FP64 oracle <= 1e-5).
+same transform planned and run alone up to rounding (the grouped kernel and
+the plain path's catalog kernels order the same arithmetic differently:
+ComplexF64 rel-L2 <= 1e-14, ComplexF32 <= 1e-6), and the oracle within
+tolerance (ComplexF64 rel-L2 <= 1e-12; ComplexF32 vs the FP64 oracle <= 1e-5).
 
 The first three tests exercised an earlier single persistent launch whose
 workgroups waited on each other through device counters.  That kernel hung
@@ -75,7 +77,7 @@ def test_batch_inverse_and_r2c_mix(gpu):
             alone = D.fft(X) if sign < 0 else D.ifft(X)
         else:
             alone = D.rfft(X)
-        tol = 1e-15 if prec == 1 else 1e-7
+        tol = 1e-14 if prec == 1 else 1e-6  # grouped vs catalog kernels: rounding only
         assert O.rel_l2(Y.to_host(), alone.to_host()) <= tol
 
 
@@ -130,7 +132,7 @@ def test_grouped_batch_small_mix(gpu):
             alone = D.fft(X) if sign < 0 else D.ifft(X)
         else:
             alone = D.rfft(X)
-        tol = 1e-15 if prec == 1 else 1e-7
+        tol = 1e-14 if prec == 1 else 1e-6  # grouped vs catalog kernels: rounding only
         assert O.rel_l2(Y.to_host(), alone.to_host()) <= tol
 
 
@@ -144,7 +146,8 @@ def test_grouped_batch_config_64_mixed(gpu):
     worst64 = worst32 = 0.0
     for t, ((shape, prec), X, Y) in enumerate(zip(cfg, ins, outs)):
         yb = Y.to_host()
-        assert O.rel_l2(yb, D.fft(X).to_host()) < 1e-15, t
+        # the grouped kernel and the plain path's catalog kernel round differently
+        assert O.rel_l2(yb, D.fft(X).to_host()) < (1e-14 if prec == 1 else 1e-6), t
         if t % 8 == 0 or t < 4:
             dt = np.complex128 if prec == 1 else np.complex64
             a = O.fill(1000 + t, shape, dt).astype(np.complex128)

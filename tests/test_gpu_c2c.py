@@ -127,7 +127,7 @@ def test_slab_plane_pipeline_matches_pencil(gpu, shape):
     assert pp.schedule()[0] == "passes"
     Fs = ps.execute(A, A.similar())
     Fp = pp.execute(A, A.similar())
-    assert O.rel_l2(Fs.to_host(), Fp.to_host()) < 1e-15
+    assert O.rel_l2(Fs.to_host(), Fp.to_host()) < 1e-14  # grouped vs catalog kernels: rounding only
     assert O.rel_l2(Fs.to_host(), O.fftn(a)) <= TOL64
     pi = D.Plan(shape, L.C2C, +1, L.F64, D.Slab(), (1, 2, 3), flags=L.FLAG_PLANE_PIPELINE)
     assert pi.schedule()[0] == "plane_pipeline"

@@ -1,0 +1,108 @@
+"""Every kernel of the compile-time catalog (dfft_fixed_*.hip) gives the
+oracle's result: each {2,3,5,7}-smooth length 16..512 along the contiguous
+axis, along a strided axis with outer slices and along a strided axis
+without, in both precisions, plus the fused R2C kernels of every half-length
+16..256, plus each FP64 strided length through both of its schedules
+(split and full LDS exchange).  ComplexF64 rel-L2 <= 1e-12; ComplexF32
+against the FP64 oracle <= 2e-6 (stated, and the worst value is printed)."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _smooth(m):
+    for p in (2, 3, 5, 7):
+        while m % p == 0:
+            m //= p
+    return m == 1
+
+
+LENGTHS = [m for m in range(16, 513) if _smooth(m)]
+HALF = [m for m in range(16, 257) if _smooth(m)]
+TOL = {1: 1e-12, 0: 2e-6}
+
+
+def _chunks(xs, k):
+    n = (len(xs) + k - 1) // k
+    return [xs[i * n:(i + 1) * n] for i in range(k)]
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+@pytest.mark.parametrize("part", range(4))
+def test_catalog_c2c_lengths(gpu, prec, part):
+    import daggerfft as D
+    from daggerfft import _lib as L
+    D.tune_clear()
+    dt = np.complex128 if prec else np.complex64
+    worst = 0.0
+    for n in _chunks(LENGTHS, 4)[part]:
+        for shape, ax in (((n, 4, 3), 0), ((3, n, 4), 1), ((4, 3, n), 2)):
+            print("C2C", "f64" if prec else "f32", shape, flush=True)
+            a = O.fill(n + ax, shape, dt)
+            A = D.DeviceArray.from_host(a)
+            p = D.Plan(shape, L.C2C, -1, prec, D.Pencil(), (1, 2, 3))
+            var = [q["variant"] for q in p.passes() if q["axis"] == ax][0]
+            assert var >= 1, (shape, var)  # the catalog kernel ran
+            F = p.execute(A, A.similar())
+            e = O.rel_l2(F.to_host().astype(np.complex128), O.fftn(a.astype(np.complex128)))
+            assert e <= TOL[prec], (shape, var, e)
+            worst = max(worst, e)
+            pi = D.Plan(shape, L.C2C, +1, prec, D.Pencil(), (1, 2, 3))
+            B = pi.execute(F, F.similar())
+            e2 = O.rel_l2(B.to_host().astype(np.complex128), a.astype(np.complex128))
+            assert e2 <= TOL[prec], (shape, "inverse", e2)
+    print("worst rel-L2 (%s): %.3e" % ("f64" if prec else "f32", worst))
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+def test_catalog_r2c_half_lengths(gpu, prec):
+    import daggerfft as D
+    from daggerfft import _lib as L
+    D.tune_clear()
+    dt = np.float64 if prec else np.float32
+    worst = 0.0
+    for h in HALF:
+        shape = (2 * h, 3, 2)
+        print("R2C", "f64" if prec else "f32", shape, flush=True)
+        a = O.fill(h, shape, dt)
+        p = D.Plan(shape, L.R2C, -1, prec, D.Pencil(), (1, 2, 3))
+        assert p.passes()[0]["variant"] == 3 and p.passes()[0]["op"] == "R2C"
+        F = D.rfft(D.DeviceArray.from_host(a))
+        e = O.rel_l2(F.to_host().astype(np.complex128), O.rfftn(a.astype(np.float64)))
+        assert e <= TOL[prec], (shape, e)
+        worst = max(worst, e)
+    print("worst rel-L2 (%s): %.3e" % ("f64" if prec else "f32", worst))
+
+
+def test_catalog_f64_strided_both_schedules(gpu, tmp_path):
+    """FP64 strided lines have a split-LDS (2) and a full-LDS (3) kernel;
+    each is forced through the schedule cache for every length."""
+    import daggerfft as D
+    from daggerfft import _lib as L
+    for variant in (2, 3):
+        for n in LENGTHS:
+            if n & (n - 1) == 0 and n >= 64 and variant == 3:
+                continue  # powers of two >= 64 have no variant-3 kernel
+            shape = (2, n, 3)
+            sc = int(math.floor(math.log2(int(np.prod(shape)))))
+            path = os.path.join(str(tmp_path), "tune.txt")
+            with open(path, "w") as f:
+                f.write("1/0/%d/2/%d - 0 8 0 %d\n" % (n, sc, variant))
+            D.tune_clear()
+            D.tune_load(path)
+            try:
+                p = D.Plan(shape, L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3))
+                got = [q["variant"] for q in p.passes() if q["axis"] == 1][0]
+                assert got == variant, (n, got)
+                a = O.fill(n, shape, np.complex128)
+                F = p.execute(D.DeviceArray.from_host(a), D.DeviceArray(shape, np.complex128))
+                e = O.rel_l2(F.to_host(), O.fftn(a))
+                assert e <= 1e-12, (n, variant, e)
+            finally:
+                D.tune_clear()

@@ -2,7 +2,9 @@
   variant 0: runtime radix schedule (any length),
   variant 1: compile-time schedule of the length (the default where built),
   variant 2: compile-time schedule, 16 values per thread, real and imaginary
-             parts exchanged through half the LDS (strided passes).
+             parts exchanged through half the LDS (strided passes),
+  variant 3: compile-time schedule, 16 values per thread (mixed radix;
+             every length is covered in test_gpu_catalog.py).
 Variants are forced through the schedule cache (dfft_tune_load)."""
 import math
 import os
@@ -64,5 +66,6 @@ def test_default_uses_fixed_schedule(gpu):
     D.tune_clear()
     p = D.Plan((512, 256, 100), L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3))
     v = [q["variant"] for q in p.passes()]
-    # 512 (contiguous) and 256 (strided, split LDS) have compile-time schedules, 100 does not
-    assert v == [1, 2, 0], v
+    # 512 (contiguous, 8 values per thread), 256 (strided, split LDS) and 100
+    # (mixed radix, 16 values per thread) all run catalog kernels
+    assert v == [1, 2, 3], v
