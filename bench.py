@@ -125,7 +125,7 @@ def serving(count):
     out["saturating"] = {k: round(v, 4) if isinstance(v, float) else v for k, v in sat.items()
                          if not k.startswith("latency")}
     out["server_stats"] = {k: st[k] for k in ("plans", "peak_running", "overtakes", "buffer_allocs",
-                                              "buffer_reuses", "buffer_evictions", "peak_bytes_held")}
+                                              "arena_chunks", "arena_waits", "peak_bytes_held")}
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
     # warm sequential: plan cache pre-built for every shape, as the server's plans are

@@ -8,7 +8,8 @@
 // Variants (AxisPass::variant):
 //   1  8 values per thread, whole complex values exchanged through LDS
 //   2  16 values per thread, real and imaginary parts exchanged in two
-//      rounds through half the LDS (strided FP64 lines of >= 256 points)
+//      rounds through half the LDS (C2C only: the fused R2C/C2R steps keep
+//      whole complex values in LDS)
 //   3  16 values per thread, whole complex values exchanged through LDS
 //      (the mixed-radix lengths: radix-3/5/7 stages leave at most 1-2 of
 //      the 16 registers idle, against 3 of 8)
@@ -45,15 +46,15 @@ struct FixedTable {
     {P, OPV, CONTIG, VAR, RL::L, fixed_tpl<RL, EMAXV>(), RL::S, RL::R,                                 \
      (axis_kernel_t)k_axis_fixed<T, OPV, CONTIG, EMAXV, SPLIT, RL, fixed_tpl<RL, EMAXV>()>}
 
-// FP64 C2C of length L: contiguous (3), strided split (2), strided (3)
-#define DFFT_C2C_F64(L)                                                \
-    DFFT_FX(double, 1, OP_C2C, 1, 3, 16, false, RadicesOf<L>),        \
-        DFFT_FX(double, 1, OP_C2C, 0, 2, 16, true, RadicesOf<L>),     \
-        DFFT_FX(double, 1, OP_C2C, 0, 3, 16, false, RadicesOf<L>)
-// FP32 C2C of length L: contiguous (3), strided (3)
-#define DFFT_C2C_F32(L)                                                \
-    DFFT_FX(float, 0, OP_C2C, 1, 3, 16, false, RadicesOf<L>),         \
-        DFFT_FX(float, 0, OP_C2C, 0, 3, 16, false, RadicesOf<L>)
+// C2C of length L, 16 values per thread: full (3) and split (2) LDS exchange,
+// contiguous and strided
+#define DFFT_C2C_16(T, P, L)                                           \
+    DFFT_FX(T, P, OP_C2C, 1, 3, 16, false, RadicesOf<L>),             \
+        DFFT_FX(T, P, OP_C2C, 1, 2, 16, true, RadicesOf<L>),          \
+        DFFT_FX(T, P, OP_C2C, 0, 3, 16, false, RadicesOf<L>),         \
+        DFFT_FX(T, P, OP_C2C, 0, 2, 16, true, RadicesOf<L>)
+#define DFFT_C2C_F64(L) DFFT_C2C_16(double, 1, L)
+#define DFFT_C2C_F32(L) DFFT_C2C_16(float, 0, L)
 // R2C of a real line of 2H points through an H-point FFT (contiguous axis)
 #define DFFT_R2C(H)                                                    \
     DFFT_FX(double, 1, OP_R2C, 1, 3, 16, false, RadicesOf<H>),        \

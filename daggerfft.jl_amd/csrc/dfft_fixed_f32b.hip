@@ -1,4 +1,4 @@
-// dfft_fixed_f32b.hip -- FP32 C2C catalog kernels, lengths 162..504
+// dfft_fixed_f32b.hip -- FP32 C2C catalog kernels, lengths 96..245
 // (every {2,3,5,7}-smooth length in the range except the powers of two
 // 64..1024, which have their own kernels in dfft_fixed_pow2.hip)
 #include "dfft_fixed.hpp"
@@ -6,14 +6,11 @@
 namespace dfft {
 
 DFFT_TABLE(g_fixed_f32b,
-    DFFT_C2C_F32(162), DFFT_C2C_F32(168), DFFT_C2C_F32(175), DFFT_C2C_F32(180), DFFT_C2C_F32(189), DFFT_C2C_F32(192),
-    DFFT_C2C_F32(196), DFFT_C2C_F32(200), DFFT_C2C_F32(210), DFFT_C2C_F32(216), DFFT_C2C_F32(224), DFFT_C2C_F32(225),
-    DFFT_C2C_F32(240), DFFT_C2C_F32(243), DFFT_C2C_F32(245), DFFT_C2C_F32(250), DFFT_C2C_F32(252), DFFT_C2C_F32(270),
-    DFFT_C2C_F32(280), DFFT_C2C_F32(288), DFFT_C2C_F32(294), DFFT_C2C_F32(300), DFFT_C2C_F32(315), DFFT_C2C_F32(320),
-    DFFT_C2C_F32(324), DFFT_C2C_F32(336), DFFT_C2C_F32(343), DFFT_C2C_F32(350), DFFT_C2C_F32(360), DFFT_C2C_F32(375),
-    DFFT_C2C_F32(378), DFFT_C2C_F32(384), DFFT_C2C_F32(392), DFFT_C2C_F32(400), DFFT_C2C_F32(405), DFFT_C2C_F32(420),
-    DFFT_C2C_F32(432), DFFT_C2C_F32(441), DFFT_C2C_F32(448), DFFT_C2C_F32(450), DFFT_C2C_F32(480), DFFT_C2C_F32(486),
-    DFFT_C2C_F32(490), DFFT_C2C_F32(500), DFFT_C2C_F32(504)
+    DFFT_C2C_F32(96), DFFT_C2C_F32(98), DFFT_C2C_F32(100), DFFT_C2C_F32(105), DFFT_C2C_F32(108), DFFT_C2C_F32(112),
+    DFFT_C2C_F32(120), DFFT_C2C_F32(125), DFFT_C2C_F32(126), DFFT_C2C_F32(135), DFFT_C2C_F32(140), DFFT_C2C_F32(144),
+    DFFT_C2C_F32(147), DFFT_C2C_F32(150), DFFT_C2C_F32(160), DFFT_C2C_F32(162), DFFT_C2C_F32(168), DFFT_C2C_F32(175),
+    DFFT_C2C_F32(180), DFFT_C2C_F32(189), DFFT_C2C_F32(192), DFFT_C2C_F32(196), DFFT_C2C_F32(200), DFFT_C2C_F32(210),
+    DFFT_C2C_F32(216), DFFT_C2C_F32(224), DFFT_C2C_F32(225), DFFT_C2C_F32(240), DFFT_C2C_F32(243), DFFT_C2C_F32(245)
 )
 
 }  // namespace dfft

@@ -1,4 +1,4 @@
-// dfft_fixed_f64b.hip -- FP64 C2C catalog kernels, lengths 162..504
+// dfft_fixed_f64b.hip -- FP64 C2C catalog kernels, lengths 96..245
 // (every {2,3,5,7}-smooth length in the range except the powers of two
 // 64..1024, which have their own kernels in dfft_fixed_pow2.hip)
 #include "dfft_fixed.hpp"
@@ -6,14 +6,11 @@
 namespace dfft {
 
 DFFT_TABLE(g_fixed_f64b,
-    DFFT_C2C_F64(162), DFFT_C2C_F64(168), DFFT_C2C_F64(175), DFFT_C2C_F64(180), DFFT_C2C_F64(189), DFFT_C2C_F64(192),
-    DFFT_C2C_F64(196), DFFT_C2C_F64(200), DFFT_C2C_F64(210), DFFT_C2C_F64(216), DFFT_C2C_F64(224), DFFT_C2C_F64(225),
-    DFFT_C2C_F64(240), DFFT_C2C_F64(243), DFFT_C2C_F64(245), DFFT_C2C_F64(250), DFFT_C2C_F64(252), DFFT_C2C_F64(270),
-    DFFT_C2C_F64(280), DFFT_C2C_F64(288), DFFT_C2C_F64(294), DFFT_C2C_F64(300), DFFT_C2C_F64(315), DFFT_C2C_F64(320),
-    DFFT_C2C_F64(324), DFFT_C2C_F64(336), DFFT_C2C_F64(343), DFFT_C2C_F64(350), DFFT_C2C_F64(360), DFFT_C2C_F64(375),
-    DFFT_C2C_F64(378), DFFT_C2C_F64(384), DFFT_C2C_F64(392), DFFT_C2C_F64(400), DFFT_C2C_F64(405), DFFT_C2C_F64(420),
-    DFFT_C2C_F64(432), DFFT_C2C_F64(441), DFFT_C2C_F64(448), DFFT_C2C_F64(450), DFFT_C2C_F64(480), DFFT_C2C_F64(486),
-    DFFT_C2C_F64(490), DFFT_C2C_F64(500), DFFT_C2C_F64(504)
+    DFFT_C2C_F64(96), DFFT_C2C_F64(98), DFFT_C2C_F64(100), DFFT_C2C_F64(105), DFFT_C2C_F64(108), DFFT_C2C_F64(112),
+    DFFT_C2C_F64(120), DFFT_C2C_F64(125), DFFT_C2C_F64(126), DFFT_C2C_F64(135), DFFT_C2C_F64(140), DFFT_C2C_F64(144),
+    DFFT_C2C_F64(147), DFFT_C2C_F64(150), DFFT_C2C_F64(160), DFFT_C2C_F64(162), DFFT_C2C_F64(168), DFFT_C2C_F64(175),
+    DFFT_C2C_F64(180), DFFT_C2C_F64(189), DFFT_C2C_F64(192), DFFT_C2C_F64(196), DFFT_C2C_F64(200), DFFT_C2C_F64(210),
+    DFFT_C2C_F64(216), DFFT_C2C_F64(224), DFFT_C2C_F64(225), DFFT_C2C_F64(240), DFFT_C2C_F64(243), DFFT_C2C_F64(245)
 )
 
 }  // namespace dfft

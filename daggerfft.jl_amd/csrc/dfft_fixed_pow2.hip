@@ -1,12 +1,13 @@
 // dfft_fixed_pow2.hip -- C2C catalog kernels of the power-of-two lengths
-// 64..1024: 8 values per thread (variant 1, both layouts) and the split-LDS
-// strided schedule (variant 2).
+// 64..1024: 8 values per thread (variant 1) and 16 values per thread with
+// the split-LDS exchange (variant 2), both layouts.
 #include "dfft_fixed.hpp"
 
 namespace dfft {
 
 #define DFFT_POW2(T, P, L)                                              \
     DFFT_FX(T, P, OP_C2C, 1, 1, 8, false, RadicesOf<L>),               \
+        DFFT_FX(T, P, OP_C2C, 1, 2, 16, true, RadicesOf<L>),           \
         DFFT_FX(T, P, OP_C2C, 0, 1, 8, false, RadicesOf<L>),           \
         DFFT_FX(T, P, OP_C2C, 0, 2, 16, true, RadicesOf<L>)
 

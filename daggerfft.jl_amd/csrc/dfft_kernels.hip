@@ -41,10 +41,11 @@ static axis_kernel_t g_axis_kernels[2][2][OP_COUNT] = {
 // ---------------------------------------------------------------------------
 // compile-time schedules (the catalog tables live in dfft_fixed_*.hip)
 // ---------------------------------------------------------------------------
-extern const FixedTable g_fixed_pow2, g_fixed_f64a, g_fixed_f64b, g_fixed_f32a, g_fixed_f32b, g_fixed_r2ca,
-    g_fixed_r2cb;
-static const FixedTable *const g_fixed_tables[] = {&g_fixed_pow2, &g_fixed_f64a, &g_fixed_f64b, &g_fixed_f32a,
-                                                   &g_fixed_f32b, &g_fixed_r2ca, &g_fixed_r2cb};
+extern const FixedTable g_fixed_pow2, g_fixed_f64a, g_fixed_f64b, g_fixed_f64c, g_fixed_f32a, g_fixed_f32b,
+    g_fixed_f32c, g_fixed_r2ca, g_fixed_r2cb;
+static const FixedTable *const g_fixed_tables[] = {&g_fixed_pow2, &g_fixed_f64a, &g_fixed_f64b, &g_fixed_f64c,
+                                                   &g_fixed_f32a, &g_fixed_f32b, &g_fixed_f32c, &g_fixed_r2ca,
+                                                   &g_fixed_r2cb};
 
 static const FixedEntry *fixed_entry(int prec, int contig, int variant, int op, int L) {
     if (variant < 1) return nullptr;

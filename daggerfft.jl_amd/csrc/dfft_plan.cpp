@@ -166,19 +166,21 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
     const int L = a.sp.L;
     const int es = pp.prec ? 16 : 8;
     // variants 1-3: a compile-time schedule of this length (see dfft_fixed.hpp).
-    // Default: the split schedule for strided FP64 lines of >= 256 points (two
-    // tiles per CU fit in LDS), else 8 values per thread where the catalog
-    // has it (powers of two), else 16.
+    // Default (measured on gfx950): the split-LDS schedule wherever the
+    // catalog has it -- half the LDS per tile doubles the tiles in flight per
+    // CU, worth 20-40% on every mixed-radix length -- except contiguous
+    // powers of two, where 8 values per thread is as fast; then 16 values.
     std::vector<int> prefs;
+    const bool pow2 = (L & (L - 1)) == 0;
     if (tune) prefs = {tune->variant};
-    else if (!a.contig && pp.prec && L >= 256) prefs = {2, 1, 3};
-    else prefs = {1, 3};
+    else if (a.contig && pow2) prefs = {1, 2, 3};
+    else prefs = {2, 1, 3};
     if (tune && tune->variant >= 2) prefs.push_back(1);
     std::vector<int> R;
     int fixed_tpl = 0;
     a.variant = 0;
     for (int want : prefs) {
-        if (want < 1 || (want == 2 && a.contig) || (grouped && want != 1)) continue;
+        if (want < 1 || (grouped && want != 1)) continue;
         if (fixed_schedule(pp.prec, a.contig, want, a.op, L, R, fixed_tpl)) {
             a.variant = want;
             break;
@@ -206,8 +208,10 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
     if (a.contig) {
         a.pad_shift = (tune && tune->pad_shift > 0) ? tune->pad_shift : (pp.prec ? 3 : 4);
         a.lds_pitch = rows + (rows >> a.pad_shift) + 1;
-        // 128-thread workgroups by default: many small tiles in flight per CU
-        int tc = (tune && tune->tc > 0) ? tune->tc : std::max(1, (contig_threads > 256 ? contig_threads : 128) / tpl);
+        // 128-thread workgroups by default (64 for the split schedule): many
+        // small tiles in flight per CU
+        const int wg = contig_threads > 256 ? contig_threads : a.variant == 2 ? 64 : 128;
+        int tc = (tune && tune->tc > 0) ? tune->tc : std::max(1, wg / tpl);
         while (tc > 1 && (int64_t)tc * tpl > max_threads) --tc;
         const int contig_lds = contig_threads > 256 ? lds_limit : std::min(lds_limit, 65536);
         while (tc > 1 && (size_t)tc * a.lds_pitch * lds_es > (size_t)contig_lds) --tc;
@@ -219,10 +223,7 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
     } else {
         a.pad_shift = 30;
         // 256-byte (FP64) / 512-byte rows; the split schedule doubles the row over outer slices
-        int tc = (tune && tune->tc > 0) ? tune->tc
-                 : a.variant == 2         ? (a.nouter > 1 ? 32 : 16)
-                 : a.variant == 3         ? 16
-                                          : (pp.prec ? 16 : 32);
+        int tc = (tune && tune->tc > 0) ? tune->tc : (a.variant >= 2 || pp.prec) ? 16 : 32;
         int64_t cols = a.nlines;
         while (tc > 1 && tc / 2 >= cols) tc /= 2;
         while (tc > 1 && (int64_t)tc * tpl > max_threads) tc /= 2;

@@ -12,11 +12,12 @@
 //              memory under the budget allow.  A request that does not fit may
 //              be overtaken by later ones that do, but the oldest such request
 //              keeps its bytes reserved, so overtaking never starves it.  An
-//              admitted request takes pooled buffers, and its launches (input
+//              admitted request takes buffers from the arena, and its launches (input
 //              fill when the caller gave no input, the plan's passes, an event)
 //              go to the lane (HIP stream) with the least outstanding work.
-//   pool       buffers are kept by size class after use; an idle buffer is
-//              freed only when the budget needs its room for another class.
+//   arena      device memory grows in chunks up to the budget and is carved
+//              into request buffers; nothing is allocated or freed on the
+//              device while requests run.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -43,90 +44,105 @@ namespace {
 
 using clk = std::chrono::steady_clock;
 
-constexpr size_t GRAIN = 2u << 20;  // pool granularity (2 MiB)
+constexpr size_t GRAIN = 2u << 20;           // allocation granularity (2 MiB)
+constexpr size_t CHUNK = (size_t)8 << 30;     // arena growth step (8 GiB)
 
-// Size classes: multiples of 2 MiB up to 8 MiB, then 2^k * {4,5,6,7}/4, so a
-// buffer is at most 25% larger than asked for and mixed sizes share classes.
-size_t size_class(size_t b) {
-    if (b == 0) return 0;
-    if (b <= 4 * GRAIN) return (b + GRAIN - 1) / GRAIN * GRAIN;
-    const int k = 63 - __builtin_clzll((unsigned long long)b);
-    const size_t base = (size_t)1 << k;
-    for (int q = 4; q <= 8; ++q) {
-        const size_t c = base / 4 * q;
-        if (c >= b) return (c + GRAIN - 1) / GRAIN * GRAIN;
-    }
-    return (b + GRAIN - 1) / GRAIN * GRAIN;
-}
+size_t round_up(size_t b) { return (b + GRAIN - 1) / GRAIN * GRAIN; }
 
 struct Buf {
     void *p = nullptr;
-    size_t cls = 0;
+    size_t bytes = 0;
+    int chunk = -1;
 };
 
-struct Pool {
+// Device memory of the server: chunks allocated as the load grows (up to the
+// budget) and kept until the server is destroyed, carved into request
+// buffers by a best-fit free list that merges neighbours on release.  No
+// device allocation or free happens while requests run, so no request ever
+// waits for an implicit device synchronisation of hipFree.
+struct Arena {
+    struct Chunk {
+        char *base;
+        size_t size;
+        std::map<size_t, size_t> free;  // offset -> length of each free range
+    };
     size_t budget = 0;
-    size_t held = 0;  // buffers allocated (busy + idle)
-    size_t busy = 0;
-    int64_t allocs = 0, reuses = 0, evictions = 0;
-    uint64_t tick = 0;
-    std::multimap<size_t, std::pair<uint64_t, void *>> idle;  // class -> (last use, pointer)
+    size_t held = 0;  // bytes of all chunks
+    size_t busy = 0;  // bytes carved out
+    int64_t allocs = 0, grows = 0, waits = 0;
+    std::vector<Chunk> chunks;
 
-    // frees the least recently used idle buffer
-    bool evict_one() {
-        if (idle.empty()) return false;
-        auto lru = idle.begin();
-        for (auto it = idle.begin(); it != idle.end(); ++it)
-            if (it->second.first < lru->second.first) lru = it;
-        (void)hipFree(lru->second.second);
-        held -= lru->first;
-        idle.erase(lru);
-        ++evictions;
+    bool carve(size_t bytes, Buf &b) {
+        int bc = -1;
+        size_t boff = 0, blen = ~(size_t)0;
+        for (size_t c = 0; c < chunks.size(); ++c)
+            for (const auto &fr : chunks[c].free)
+                if (fr.second >= bytes && fr.second < blen) {
+                    bc = (int)c;
+                    boff = fr.first;
+                    blen = fr.second;
+                }
+        if (bc < 0) return false;
+        auto &fl = chunks[bc].free;
+        fl.erase(boff);
+        if (blen > bytes) fl.emplace(boff + bytes, blen - bytes);
+        b.p = chunks[bc].base + boff;
+        b.bytes = bytes;
+        b.chunk = bc;
+        busy += bytes;
+        ++allocs;
         return true;
     }
-    // a buffer of class `cls`; `extra` = other bytes the budget must leave room for
-    int acquire(size_t cls, size_t extra, Buf &b) {
+    // a buffer of at least `bytes`; `other` = bytes the budget must also hold (plans)
+    int acquire(size_t bytes, size_t other, Buf &b) {
         b = Buf{};
-        if (cls == 0) return DFFT_OK;
-        auto it = idle.find(cls);
-        if (it != idle.end()) {
-            b.p = it->second.second;
-            b.cls = cls;
-            idle.erase(it);
-            busy += cls;
-            ++reuses;
-            return DFFT_OK;
-        }
-        while (held + cls + extra > budget && evict_one()) {
-        }
-        if (held + cls + extra > budget) return DFFT_ERR_BUDGET;
+        if (bytes == 0) return DFFT_OK;
+        bytes = round_up(bytes);
+        if (carve(bytes, b)) return DFFT_OK;
+        const size_t room = budget > held + other ? budget - held - other : 0;
+        size_t want = std::min(room, std::max(bytes, CHUNK)) / GRAIN * GRAIN;
+        if (want < bytes) return DFFT_ERR_BUDGET;
         void *p = nullptr;
-        if (hipMalloc(&p, cls) != hipSuccess) {
+        if (hipMalloc(&p, want) != hipSuccess) {
             (void)hipGetLastError();
-            while (evict_one()) {
-            }
-            if (hipMalloc(&p, cls) != hipSuccess) {
+            want = bytes;
+            if (hipMalloc(&p, want) != hipSuccess) {
                 (void)hipGetLastError();
                 return DFFT_ERR_OOM;
             }
         }
-        b.p = p;
-        b.cls = cls;
-        held += cls;
-        busy += cls;
-        ++allocs;
-        return DFFT_OK;
+        chunks.push_back(Chunk{static_cast<char *>(p), want, {{0, want}}});
+        held += want;
+        ++grows;
+        return carve(bytes, b) ? DFFT_OK : DFFT_ERR_OOM;
     }
     void release(Buf &b) {
         if (!b.p) return;
-        busy -= b.cls;
-        idle.emplace(b.cls, std::make_pair(++tick, b.p));
+        Chunk &ch = chunks[b.chunk];
+        size_t off = (size_t)(static_cast<char *>(b.p) - ch.base), len = b.bytes;
+        auto &fl = ch.free;
+        auto next = fl.lower_bound(off);
+        if (next != fl.end() && off + len == next->first) {
+            len += next->second;
+            next = fl.erase(next);
+        }
+        if (next != fl.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                off = prev->first;
+                len += prev->second;
+                fl.erase(prev);
+            }
+        }
+        fl.emplace(off, len);
+        busy -= b.bytes;
         b = Buf{};
     }
+    // frees every chunk; only when nothing is carved out (no request holds memory)
     void free_all() {
-        for (auto &kv : idle) (void)hipFree(kv.second.second);
-        idle.clear();
-        held = busy;
+        for (auto &c : chunks) (void)hipFree(c.base);
+        chunks.clear();
+        held = 0;
     }
 };
 
@@ -141,7 +157,7 @@ struct Req {
     double t_arrival = 0, t_admit = 0, t_done = 0;
     const dfft_plan *plan = nullptr;
     size_t in_bytes = 0, out_bytes = 0, work_bytes = 0;
-    size_t cls_in = 0, cls_out = 0, cls_work = 0;  // pool classes this request takes
+    size_t cls_in = 0, cls_out = 0, cls_work = 0;  // server buffer bytes this request takes
     Buf in, out, work;
     const void *result = nullptr;
     size_t result_bytes = 0;
@@ -163,9 +179,10 @@ struct dfft_server {
     std::condition_variable cv_sched;  // scheduler: new work, released memory, stop
     std::condition_variable cv_done;   // clients: a request finished
     bool stop = false;
-    Pool pool;
+    Arena pool;
     std::map<PlanKey, dfft_plan *> plans;
     size_t plan_bytes = 0;
+    size_t plan_reserve = 0;  // budget kept out of the arena for plans built later
     std::unordered_map<int64_t, Req *> reqs;
     std::deque<Req *> queue;
     std::vector<Req *> running;
@@ -290,15 +307,23 @@ bool dfft_server::admit() {
             ++it;
             continue;
         }
-        // buffers (evicting idle ones of other classes as needed)
-        const size_t extra = plan_bytes + reserved;
-        int rc = pool.acquire(q->cls_in, extra + q->cls_out + q->cls_work, q->in);
-        if (rc == DFFT_OK) rc = pool.acquire(q->cls_out, extra + q->cls_work, q->out);
-        if (rc == DFFT_OK) rc = pool.acquire(q->cls_work, extra, q->work);
-        if (rc != DFFT_OK) {
+        int rc = DFFT_OK;
+        for (int attempt = 0; attempt < 2; ++attempt) {
+            const size_t other = std::max(plan_bytes, plan_reserve);
+            rc = pool.acquire(q->cls_in, other, q->in);
+            if (rc == DFFT_OK) rc = pool.acquire(q->cls_out, other, q->out);
+            if (rc == DFFT_OK) rc = pool.acquire(q->cls_work, other, q->work);
+            if (rc == DFFT_OK) break;
             pool.release(q->in);
             pool.release(q->out);
             pool.release(q->work);
+            // no contiguous room although the budget has it: with nothing
+            // carved out, return the chunks and lay the arena out afresh
+            if (attempt == 0 && pool.busy == 0 && running.empty()) pool.free_all();
+            else break;
+        }
+        if (rc != DFFT_OK) {
+            ++pool.waits;
             if (rc == DFFT_ERR_OOM && running.empty() && pool.busy == 0) {
                 // nothing else holds device memory: it can never fit
                 it = queue.erase(it);
@@ -383,9 +408,7 @@ static int get_plan(dfft_server *s, const dfft_request *r, const dfft_plan **out
     std::lock_guard<std::mutex> g(s->mu);
     s->plans.emplace(key, p);
     s->plan_bytes += dfft::plan_resident_bytes(p);
-    // plans count against the budget: make room among idle buffers
-    while (s->pool.held + s->plan_bytes > s->pool.budget && s->pool.evict_one()) {
-    }
+
     *out = p;
     return DFFT_OK;
 }
@@ -439,6 +462,7 @@ int dfft_server_create(dfft_server **server, const dfft_server_config *cfg) {
         return dfft::hip_fail(e, "hipMemGetInfo");
     }
     s->pool.budget = c.hbm_budget ? c.hbm_budget : free_b / 10 * 9;
+    s->plan_reserve = std::min<size_t>(s->pool.budget / 64, (size_t)64 << 20);
     s->max_inflight = c.max_inflight ? c.max_inflight : 32;
     const int nl = c.lanes ? c.lanes : 4;
     for (int l = 0; l < nl; ++l) {
@@ -498,9 +522,9 @@ int dfft_server_submit(dfft_server *s, const dfft_request *r, int64_t *ticket) {
     // server buffers: the input when the caller gave none (C2C fills its
     // output and runs in place), the output when the caller gave none
     const bool inplace_c2c = r->kind == DFFT_C2C && !r->d_in;
-    if (!r->d_in && !inplace_c2c) q->cls_in = size_class(q->in_bytes);
-    if (!r->d_out) q->cls_out = size_class(q->out_bytes);
-    q->cls_work = size_class(q->work_bytes);
+    if (!r->d_in && !inplace_c2c) q->cls_in = round_up(q->in_bytes);
+    if (!r->d_out) q->cls_out = round_up(q->out_bytes);
+    q->cls_work = round_up(q->work_bytes);
     std::lock_guard<std::mutex> g(s->mu);
     if (q->need() + s->plan_bytes > s->pool.budget) {
         delete q;
@@ -610,8 +634,8 @@ int dfft_server_stats_get(dfft_server *s, dfft_server_stats *st) {
     st->plans = (int64_t)s->plans.size();
     st->overtakes = s->overtakes;
     st->buffer_allocs = s->pool.allocs;
-    st->buffer_reuses = s->pool.reuses;
-    st->buffer_evictions = s->pool.evictions;
+    st->arena_chunks = s->pool.grows;
+    st->arena_waits = s->pool.waits;
     st->budget = s->pool.budget;
     st->bytes_held = s->pool.held + s->plan_bytes;
     st->bytes_busy = s->pool.busy;

@@ -74,7 +74,7 @@ class dfft_request_info(ctypes.Structure):
 class dfft_server_stats(ctypes.Structure):
     _fields_ = [(k, ctypes.c_int64) for k in ("submitted", "completed", "failed", "queued", "running",
                                               "peak_running", "plans", "overtakes", "buffer_allocs",
-                                              "buffer_reuses", "buffer_evictions")] + \
+                                              "arena_chunks", "arena_waits")] + \
                [(k, ctypes.c_size_t) for k in ("budget", "bytes_held", "bytes_busy", "peak_bytes_held")]
 
 

@@ -1,0 +1,59 @@
+"""Regenerates the schedule cache shipped beside the library
+(wisdom/<gfx arch>.txt, loaded at the first plan creation): runs the
+autotuner on the device over the transforms this library is measured on and
+saves the fastest schedule of every axis pass.
+
+  python daggerfft.jl_amd/wisdom/make_wisdom.py [out.txt] [--quick]
+
+Shapes: 512^3 ComplexF64 (full search), 128^3 ComplexF64, 256^3 R2C Float64,
+384x240x210 ComplexF64, the DCT-II/III configurations, and every distinct
+request shape of the mixed serving trace (cubic 64..512, C2C and R2C, both
+precisions).  Progress is printed per shape.
+"""
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+
+
+def main():
+    args = [a for a in sys.argv[1:] if not a.startswith("--")]
+    quick = "--quick" in sys.argv
+    out = args[0] if args else None
+    import torch
+    import daggerfft as D
+    from daggerfft import _lib as L
+    from daggerfft import serve as S
+    torch.cuda.init()
+    D.tune_clear()
+    jobs = [((512, 512, 512), L.C2C, L.F64, None, 2), ((128, 128, 128), L.C2C, L.F64, None, 1),
+            ((256, 256, 256), L.R2C, L.F64, None, 1), ((384, 240, 210), L.C2C, L.F64, None, 1),
+            ((256, 256), L.R2R, L.F64, (D.REDFT10, D.REDFT10), 1),
+            ((128, 128, 128), L.R2R, L.F64, (D.REDFT10,) * 3, 1)]
+    seen = set()
+    for q in S.serving_trace():
+        key = (q.shape, q.kind, q.precision)
+        if key not in seen:
+            seen.add(key)
+            jobs.append((q.shape, q.kind, q.precision, None, 1))
+    if quick:
+        jobs = jobs[:8]
+    t0 = time.time()
+    for i, (shape, kind, prec, r2r, level) in enumerate(jobs):
+        D.autotune(shape, kind=kind, sign=-1, precision=prec, r2r_kinds=r2r, level=level)
+        print("[%d/%d] %s %s %s  %.1fs  entries %d" % (i + 1, len(jobs), "x".join(map(str, shape)),
+                                                     ("C2C", "R2C", "C2R", "R2R")[kind], "f64" if prec else "f32",
+                                                     time.time() - t0, D.tune_entries()), flush=True)
+    if out is None:
+        import subprocess
+        arch = subprocess.run(["rocminfo"], capture_output=True, text=True).stdout
+        arch = next((w for w in arch.split() if w.startswith("gfx")), "gfx950")
+        out = os.path.join(HERE, arch + ".txt")
+    D.tune_save(out)
+    print("saved", D.tune_entries(), "entries to", out, flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -31,7 +31,7 @@
  *   dfft_batch_*         no reference counterpart: many independent transforms
  *                        per launch (one launch per axis-pass level)
  *   dfft_server_*        no reference counterpart: continuous-batching request
- *                        loop with resident plans and pooled buffers
+ *                        loop with resident plans and a device memory arena
  */
 #ifndef DAGGERFFT_HIP_H
 #define DAGGERFFT_HIP_H
@@ -246,14 +246,14 @@ int dfft_batch_destroy(dfft_batch *batch);
  * others run overlaps on the device -- and retires it when its completion
  * event fires.  Plans are built once per (shape, kind, sign, precision,
  * decomp, flags), at the first submission of that shape, and kept until
- * dfft_server_destroy.  Device buffers are pooled by size class and reused;
- * idle buffers are freed only when the budget needs the room for another
- * class.  A request's result is the same as dfft_execute of the same plan on
- * the same input (the same kernels run). */
+ * dfft_server_destroy.  Device memory is an arena that grows in chunks up to
+ * the budget and is carved into request buffers; nothing is allocated or
+ * freed on the device while requests run.  A request's result is the same
+ * as dfft_execute of the same plan on the same input (the same kernels run). */
 typedef struct dfft_server dfft_server;
 
 typedef struct {
-    size_t hbm_budget;      /* device bytes the server may hold: pooled buffers + plans (0: 90% of free memory) */
+    size_t hbm_budget;      /* device bytes the server may hold: arena + plans (0: 90% of free memory) */
     int max_inflight;       /* requests admitted and not yet retired (0: 32) */
     int lanes;              /* HIP streams the requests are spread over (0: 4) */
     uint32_t flags;         /* reserved, must be 0 */
@@ -298,9 +298,11 @@ typedef struct {
     int64_t queued, running, peak_running;
     int64_t plans;          /* resident plans */
     int64_t overtakes;      /* admissions that overtook a request waiting for memory */
-    int64_t buffer_allocs, buffer_reuses, buffer_evictions;
+    int64_t buffer_allocs;  /* request buffers carved from the arena */
+    int64_t arena_chunks;   /* device allocations of the arena (it grows in 8 GiB chunks) */
+    int64_t arena_waits;    /* admissions deferred for want of a contiguous range */
     size_t budget;
-    size_t bytes_held;      /* pooled buffers (busy + idle) + plans */
+    size_t bytes_held;      /* arena chunks + plans */
     size_t bytes_busy;      /* buffers of running requests and of unreleased results */
     size_t peak_bytes_held;
 } dfft_server_stats;

@@ -101,7 +101,7 @@ def test_budget_admission_queues_and_overtakes(gpu):
         st = srv.stats()
     assert st["completed"] == len(order)
     assert st["peak_bytes_held"] <= budget, st
-    assert st["buffer_reuses"] > 0
+    assert st["arena_chunks"] >= 1 and st["buffer_allocs"] >= len(order)
     for q, y in zip(order, outs):
         assert O.rel_l2(y, _alone(q)) <= 1e-12
 

@@ -66,6 +66,6 @@ def test_default_uses_fixed_schedule(gpu):
     D.tune_clear()
     p = D.Plan((512, 256, 100), L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3))
     v = [q["variant"] for q in p.passes()]
-    # 512 (contiguous, 8 values per thread), 256 (strided, split LDS) and 100
-    # (mixed radix, 16 values per thread) all run catalog kernels
-    assert v == [1, 2, 3], v
+    # 512 (contiguous, 8 values per thread), 256 and 100 (strided, split LDS)
+    # all run catalog kernels
+    assert v == [1, 2, 2], v
