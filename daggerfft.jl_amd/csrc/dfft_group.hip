@@ -182,6 +182,72 @@ int group_launch(const GroupExec *g, void *const *slots, int nslots, hipStream_t
     return DFFT_OK;
 }
 
+// The launches of a group recorded once as a hipGraph (kernel nodes in a
+// chain) and replayed; re-recorded when the slot pointers change.
+struct GroupGraph {
+    std::mutex mu;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<void *> slots;
+};
+
+static void graph_reset(GroupGraph *gg) {
+    if (gg->exec) (void)hipGraphExecDestroy(gg->exec);
+    if (gg->graph) (void)hipGraphDestroy(gg->graph);
+    gg->exec = nullptr;
+    gg->graph = nullptr;
+    gg->slots.clear();
+}
+
+GroupGraph *group_graph_new() { return new (std::nothrow) GroupGraph(); }
+
+void group_graph_destroy(GroupGraph *gg) {
+    if (!gg) return;
+    graph_reset(gg);
+    delete gg;
+}
+
+int group_launch_graph(const GroupExec *g, GroupGraph *gg, void *const *slots, int nslots, hipStream_t st) {
+    if (nslots > GROUP_MAX_SLOTS) return DFFT_ERR_ARG;
+    std::lock_guard<std::mutex> lk(gg->mu);
+    std::vector<void *> cur(slots, slots + nslots);
+    if (!gg->exec || gg->slots != cur) {
+        graph_reset(gg);
+        hipGraph_t graph;
+        if (hipError_t e = hipGraphCreate(&graph, 0)) return hip_fail(e, "hipGraphCreate");
+        hipGraphNode_t prev = nullptr;
+        for (const auto &L : g->launches) {
+            GroupArgs a{};
+            for (int i = 0; i < nslots; ++i) a.slots[i] = slots[i];
+            a.items = g->d_items + L.first_item;
+            a.block_item = g->d_map + L.map_off;
+            void *kargs[] = {&a};
+            hipKernelNodeParams kp{};
+            kp.func = const_cast<void *>(group_kernel(L.threads));
+            kp.gridDim = dim3((unsigned)L.nblocks);
+            kp.blockDim = dim3(L.threads);
+            kp.sharedMemBytes = (unsigned)L.lds;
+            kp.kernelParams = kargs;
+            hipGraphNode_t node;
+            if (hipError_t e = hipGraphAddKernelNode(&node, graph, prev ? &prev : nullptr, prev ? 1 : 0, &kp)) {
+                (void)hipGraphDestroy(graph);
+                return hip_fail(e, "hipGraphAddKernelNode");
+            }
+            prev = node;
+        }
+        hipGraphExec_t exec;
+        if (hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0)) {
+            (void)hipGraphDestroy(graph);
+            return hip_fail(e, "hipGraphInstantiate");
+        }
+        gg->graph = graph;
+        gg->exec = exec;
+        gg->slots = cur;
+    }
+    if (hipError_t e = hipGraphLaunch(gg->exec, st)) return hip_fail(e, "hipGraphLaunch");
+    return DFFT_OK;
+}
+
 int group_num_launches(const GroupExec *g) { return (int)g->launches.size(); }
 
 int64_t group_blocks(const GroupExec *g) {
@@ -217,6 +283,7 @@ struct dfft_batch {
     int count = 0;
     std::vector<dfft_batch_desc> descs;
     GroupExec *g = nullptr;
+    GroupGraph *graph = nullptr;  // DFFT_FLAG_GRAPH: the launches replayed as one hipGraph
     void *d_tw64 = nullptr, *d_tw32 = nullptr;
     size_t tw64_bytes = 0, tw32_bytes = 0;
     double alg_bytes = 0;
@@ -224,6 +291,7 @@ struct dfft_batch {
 
 static void batch_free(dfft_batch *b) {
     if (!b) return;
+    group_graph_destroy(b->graph);
     group_destroy(b->g);
     if (b->d_tw64) (void)hipFree(b->d_tw64);
     if (b->d_tw32) (void)hipFree(b->d_tw32);
@@ -329,6 +397,7 @@ int dfft_batch_create(dfft_batch **batch, int count, const dfft_batch_desc *desc
         }
     }
     rc = group_create(&b->g, levels);
+    if (rc == DFFT_OK && (flags & DFFT_FLAG_GRAPH) && !(b->graph = group_graph_new())) rc = DFFT_ERR_OOM;
     if (rc != DFFT_OK) {
         batch_free(b);
         return rc;
@@ -346,6 +415,7 @@ int dfft_batch_execute(dfft_batch *b, void *const *d_in, void *const *d_out, voi
         slots[2 * t] = d_in[t];
         slots[2 * t + 1] = d_out[t];
     }
+    if (b->graph) return group_launch_graph(b->g, b->graph, slots.data(), (int)slots.size(), (hipStream_t)stream);
     return group_launch(b->g, slots.data(), (int)slots.size(), (hipStream_t)stream);
 }
 

@@ -31,6 +31,12 @@ struct GroupExec;
 int group_create(GroupExec **out, const std::vector<std::vector<GroupItem>> &levels);
 int group_launch(const GroupExec *g, void *const *slots, int nslots, hipStream_t st);
 void group_destroy(GroupExec *g);
+struct GroupGraph;
+GroupGraph *group_graph_new();
+void group_graph_destroy(GroupGraph *gg);
+// the group's launches as one hipGraph replay (recorded on first use and
+// whenever the slot pointers change)
+int group_launch_graph(const GroupExec *g, GroupGraph *gg, void *const *slots, int nslots, hipStream_t st);
 int group_num_launches(const GroupExec *g);
 int64_t group_blocks(const GroupExec *g);
 GroupItem make_group_item(const PassPlan &pp, int in_slot, int64_t in_off, int out_slot, int64_t out_off);

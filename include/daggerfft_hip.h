@@ -208,7 +208,10 @@ int dfft_tune_entries(int *count);
  * (the reference runs one cuFFT plan per chunk, src/fftgpu.jl:118-124).
  * Each transform is rank 3 over dims (0,1,2); C2C (either sign) or R2C
  * (forward, even n[0]).  Launch l runs axis pass l of every transform, so a
- * whole batch takes three launches.  Results equal each transform run alone. */
+ * whole batch takes three launches; with DFFT_FLAG_GRAPH at creation they are
+ * recorded once as a hipGraph and replayed by every execute (re-recorded when
+ * the buffer pointers change).  Results equal each transform run alone up to
+ * rounding (the batch kernel follows the runtime schedule). */
 typedef struct {
     int64_t n[3];       /* size of this 3D transform (n[0] fastest) */
     int kind;           /* DFFT_C2C or DFFT_R2C */

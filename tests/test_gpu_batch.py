@@ -158,3 +158,59 @@ def test_grouped_batch_config_64_mixed(gpu):
                 worst32 = max(worst32, e)
     print("grouped batch of 64: worst rel-L2 f64 %.3e, f32 %.3e, workgroups %d" % (worst64, worst32, b.tickets))
     assert worst64 <= 1e-12 and worst32 <= 1e-5
+
+
+def test_grouped_batch_graph_replays(gpu):
+    """With DFFT_FLAG_GRAPH the batch's launches are recorded once as a
+    hipGraph and replayed: every replay -- new input data in the same
+    buffers, the program's own eager work and synchronisations in between,
+    then a second set of buffers (re-recorded) -- equals the eager launches
+    bit for bit."""
+    import ctypes
+
+    import torch
+
+    import daggerfft as D
+    from daggerfft import _lib as L
+    from daggerfft.batch import Batch
+    descs = [((32, 48, 20), L.C2C, -1, 1), ((64, 30, 14), L.R2C, -1, 1), ((16, 16, 16), L.C2C, +1, 0),
+             ((40, 21, 10), L.R2C, -1, 0), ((96, 32, 48), L.C2C, -1, 1)]
+    eager = Batch(descs)
+    graph = Batch(descs, flags=L.FLAG_GRAPH)
+
+    def buffers(seed0):
+        ins, oe, og = [], [], []
+        for t, (shape, kind, sign, prec) in enumerate(descs):
+            cplx = kind == L.C2C
+            dt = (np.complex128 if prec else np.complex64) if cplx else (np.float64 if prec else np.float32)
+            ins.append(D.rand(shape, dt, seed=seed0 + t))
+            oshape = shape if cplx else (shape[0] // 2 + 1,) + tuple(shape[1:])
+            odt = np.complex128 if prec else np.complex64
+            oe.append(D.DeviceArray(oshape, odt))
+            og.append(D.DeviceArray(oshape, odt))
+        return ins, oe, og
+
+    def refill(ins, seed0):
+        for t, X in enumerate(ins):
+            L.check(L.lib().dfft_fill_uniform(ctypes.c_void_p(X.ptr), X.size, 1 if X.is_complex else 0,
+                                              X.precision, ctypes.c_uint64(seed0 + t),
+                                              D._stream_handle(None)))
+
+    scratch = torch.rand(1 << 20, device="cuda")
+    for bufset in range(2):
+        ins, oe, og = buffers(70 + 10 * bufset)
+        for rep in range(3):
+            refill(ins, 100 * (bufset + 1) + rep)
+            eager.execute(ins, oe)
+            torch.cuda.synchronize()
+            scratch.mul_(1.5).add_(0.25)  # eager work between replays
+            graph.execute(ins, og)
+            scratch.sum().item()
+            torch.cuda.synchronize()
+            for t, (a, b) in enumerate(zip(oe, og)):
+                assert torch.equal(a.tensor, b.tensor), (bufset, rep, t)
+    # and the replays are right, not merely repeatable
+    for t, ((shape, kind, sign, prec), X, Y) in enumerate(zip(descs, ins, og)):
+        x = X.to_host().astype(np.complex128 if kind == L.C2C else np.float64)
+        ref = (O.fftn(x) if sign < 0 else O.fftn(x, inverse=True)) if kind == L.C2C else O.rfftn(x)
+        assert O.rel_l2(Y.to_host().astype(np.complex128), ref) <= (1e-12 if prec else 1e-5), t
