@@ -115,18 +115,42 @@ struct TileCtx {
             else x = conj(gload(inc + ib + (int64_t)(L - p) * a.in_sm, a.nt));
             if (p == 0 || 2 * p == L) x.y = 0;
             return conj(x);
-        } else if constexpr (OP == OP_DCT2) {
+        } else if constexpr (OP == OP_DCT2 || OP == OP_DST2) {
             if (!valid) return cx<T>{0, 0};
             const int L = a.sp.L;
             int q = (p < (L + 1) / 2) ? 2 * p : 2 * (L - 1 - p) + 1;
-            return cx<T>{gload(inr + ib + (int64_t)q * a.in_sm, a.nt), (T)0};
-        } else {  // OP_DCT3
+            T x = gload(inr + ib + (int64_t)q * a.in_sm, a.nt);
+            if constexpr (OP == OP_DST2) x = (q & 1) ? -x : x;
+            return cx<T>{x, (T)0};
+        } else if constexpr (OP == OP_DCT3 || OP == OP_DST3) {
             if (!valid) return cx<T>{0, 0};
             const int L = a.sp.L;
-            T xp = gload(inr + ib + (int64_t)p * a.in_sm, a.nt);
-            T xr = p == 0 ? (T)0 : gload(inr + ib + (int64_t)(L - p) * a.in_sm, a.nt);
+            // DST-III reads the line reversed: x'_p = x_{L-1-p}
+            const int ip = OP == OP_DCT3 ? p : L - 1 - p;
+            const int ir = OP == OP_DCT3 ? L - p : p - 1;
+            T xp = gload(inr + ib + (int64_t)ip * a.in_sm, a.nt);
+            T xr = p == 0 ? (T)0 : gload(inr + ib + (int64_t)ir * a.in_sm, a.nt);
             // conj( exp(+i pi p / 2L) (x_p - i x_{L-p}) ) = w_p (x_p + i x_{L-p})
             return cmul(post()[p], cx<T>{xp, xr});
+        } else if constexpr (OP == OP_DHT) {
+            if (!valid) return cx<T>{0, 0};
+            return cx<T>{gload(inr + ib + (int64_t)p * a.in_sm, a.nt), (T)0};
+        } else if constexpr (OP == OP_DCT1) {
+            // even extension x_0 .. x_{n-1}, x_{n-2} .. x_1
+            if (!valid) return cx<T>{0, 0};
+            const int n = a.nlog, q = p < n ? p : a.sp.L - p;
+            return cx<T>{gload(inr + ib + (int64_t)q * a.in_sm, a.nt), (T)0};
+        } else if constexpr (OP == OP_DST1) {
+            // odd extension 0, x_0 .. x_{n-1}, 0, -x_{n-1} .. -x_0
+            if (!valid) return cx<T>{0, 0};
+            const int n = a.nlog;
+            if (p == 0 || p == n + 1) return cx<T>{0, 0};
+            if (p <= n) return cx<T>{gload(inr + ib + (int64_t)(p - 1) * a.in_sm, a.nt), (T)0};
+            return cx<T>{-gload(inr + ib + (int64_t)(a.sp.L - p - 1) * a.in_sm, a.nt), (T)0};
+        } else {  // OP_DCT4, OP_DST4: x_p e^{-i pi p / 2n}, zero-padded to 2n
+            if (!valid || p >= a.nlog) return cx<T>{0, 0};
+            T x = gload(inr + ib + (int64_t)p * a.in_sm, a.nt);
+            return scl(post()[2 * p], x);
         }
     }
 
@@ -153,13 +177,26 @@ struct TileCtx {
                 }
             } else if constexpr (OP == OP_C2R_FULL) {
                 gstore(outr + ob + (int64_t)p * a.out_sm, v.x * s, a.nt);
-            } else if constexpr (OP == OP_DCT2) {
+            } else if constexpr (OP == OP_DCT2 || OP == OP_DST2) {
                 cx<T> w = post()[p];
-                gstore(outr + ob + (int64_t)p * a.out_sm, (T)2 * (w.x * v.x - w.y * v.y) * s, a.nt);
-            } else {  // OP_DCT3
+                const int q = OP == OP_DCT2 ? p : a.sp.L - 1 - p;
+                gstore(outr + ob + (int64_t)q * a.out_sm, (T)2 * (w.x * v.x - w.y * v.y) * s, a.nt);
+            } else if constexpr (OP == OP_DCT3 || OP == OP_DST3) {
                 const int L = a.sp.L;
                 int q = (p < (L + 1) / 2) ? 2 * p : 2 * (L - 1 - p) + 1;
-                gstore(outr + ob + (int64_t)q * a.out_sm, v.x * s, a.nt);
+                const T y = (OP == OP_DST3 && (q & 1)) ? -v.x : v.x;
+                gstore(outr + ob + (int64_t)q * a.out_sm, y * s, a.nt);
+            } else if constexpr (OP == OP_DHT) {
+                gstore(outr + ob + (int64_t)p * a.out_sm, (v.x - v.y) * s, a.nt);
+            } else if constexpr (OP == OP_DCT1) {
+                if (p < a.nlog) gstore(outr + ob + (int64_t)p * a.out_sm, v.x * s, a.nt);
+            } else if constexpr (OP == OP_DST1) {
+                if (p >= 1 && p <= a.nlog) gstore(outr + ob + (int64_t)(p - 1) * a.out_sm, -v.y * s, a.nt);
+            } else {  // OP_DCT4, OP_DST4
+                if (p < a.nlog) {
+                    const cx<T> z = cmul(post()[2 * p + 1], v);
+                    gstore(outr + ob + (int64_t)p * a.out_sm, (T)2 * (OP == OP_DCT4 ? z.x : -z.y) * s, a.nt);
+                }
             }
         }
     }

@@ -1,6 +1,6 @@
 // dfft_kernels.hip -- HIP kernels of libdaggerfft_hip for gfx950.
 //
-//   k_axis      one Stockham axis pass per launch (C2C / R2C / C2R / DCT-II / DCT-III)
+//   k_axis      one Stockham axis pass per launch (C2C / R2C / C2R / every r2r kind)
 //   k_fill      SplitMix64 counter-based uniform[0,1) input generator
 //   k_box       box gather / scatter / copy (pack_kernel!, unpack_kernel!,
 //               local_copy_kernel!; src/fftgpu.jl:452-495), coalesced along axis 0
@@ -30,7 +30,10 @@ __global__ __launch_bounds__(1024) void k_axis(AxisPass a) {
     {(axis_kernel_t)k_axis<T, OP_C2C, CONTIG, 8>, (axis_kernel_t)k_axis<T, OP_R2C, CONTIG, 8>, \
      (axis_kernel_t)k_axis<T, OP_R2C_FULL, CONTIG, 8>, (axis_kernel_t)k_axis<T, OP_C2R, CONTIG, 8>, \
      (axis_kernel_t)k_axis<T, OP_C2R_FULL, CONTIG, 8>, (axis_kernel_t)k_axis<T, OP_DCT2, CONTIG, 8>, \
-     (axis_kernel_t)k_axis<T, OP_DCT3, CONTIG, 8>}
+     (axis_kernel_t)k_axis<T, OP_DCT3, CONTIG, 8>, (axis_kernel_t)k_axis<T, OP_DHT, CONTIG, 8>,   \
+     (axis_kernel_t)k_axis<T, OP_DST2, CONTIG, 8>, (axis_kernel_t)k_axis<T, OP_DST3, CONTIG, 8>,  \
+     (axis_kernel_t)k_axis<T, OP_DCT1, CONTIG, 8>, (axis_kernel_t)k_axis<T, OP_DST1, CONTIG, 8>,  \
+     (axis_kernel_t)k_axis<T, OP_DCT4, CONTIG, 8>, (axis_kernel_t)k_axis<T, OP_DST4, CONTIG, 8>}
 
 // runtime-schedule kernels: [precision][contig][op]
 static axis_kernel_t g_axis_kernels[2][2][OP_COUNT] = {

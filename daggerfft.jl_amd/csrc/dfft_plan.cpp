@@ -97,10 +97,15 @@ static void build_twiddles(StagePlan &sp, int op, int nlog, std::vector<long dou
         // w^k = exp(-2 pi i k / (2H)), k = 0..H
         sp.post = (int)(tab.size() / 2);
         for (int k = 0; k <= sp.L; ++k) push_w(tab, k, 2 * (int64_t)sp.L);
-    } else if (op == OP_DCT2 || op == OP_DCT3) {
+    } else if (op == OP_DCT2 || op == OP_DCT3 || op == OP_DST2 || op == OP_DST3) {
         // exp(-i pi k / 2L), k = 0..L-1
         sp.post = (int)(tab.size() / 2);
         for (int k = 0; k < sp.L; ++k) push_w(tab, k, 4 * (int64_t)sp.L);
+    } else if (op == OP_DCT4 || op == OP_DST4) {
+        // exp(-2 pi i m / 4L), m = 0..L-1 (L = 2n): pre-twiddle e^{-i pi p/2n} at
+        // m = 2p, post-twiddle e^{-i pi (2k+1)/4n} at m = 2k+1
+        sp.post = (int)(tab.size() / 2);
+        for (int m = 0; m < sp.L; ++m) push_w(tab, m, 4 * (int64_t)sp.L);
     }
     (void)nlog;
 }
@@ -319,6 +324,13 @@ static const char *op_name(int op) {
     case OP_C2R_FULL: return "C2R_FULL";
     case OP_DCT2: return "DCT2";
     case OP_DCT3: return "DCT3";
+    case OP_DHT: return "DHT";
+    case OP_DST2: return "DST2";
+    case OP_DST3: return "DST3";
+    case OP_DCT1: return "DCT1";
+    case OP_DST1: return "DST1";
+    case OP_DCT4: return "DCT4";
+    case OP_DST4: return "DST4";
     }
     return "?";
 }
@@ -421,13 +433,26 @@ int build_plan(dfft_plan *p) {
         }
         for (size_t i = 0; i < order.size(); ++i) {
             const int ax = order[i];
-            const int L = (int)n[ax];
-            int op;
-            if (kinds[i] == DFFT_REDFT10) op = OP_DCT2;
-            else if (kinds[i] == DFFT_REDFT01) op = OP_DCT3;
-            else return DFFT_ERR_UNSUPPORTED;
-            const double sc = (p->flags & DFFT_FLAG_NORMALIZE_R2R) ? 1.0 / r2r_logical(kinds[i], L) : 1.0;
-            add(ax, op, L, L, n, n, i == 0 ? BUF_IN : BUF_OUT, BUF_OUT, sc, false, 2.0 * N * es_r);
+            const int nl = (int)n[ax];
+            int op, L;  // fused op and the length of its complex FFT
+            switch (kinds[i]) {
+            case DFFT_REDFT10: op = OP_DCT2; L = nl; break;
+            case DFFT_REDFT01: op = OP_DCT3; L = nl; break;
+            case DFFT_RODFT10: op = OP_DST2; L = nl; break;
+            case DFFT_RODFT01: op = OP_DST3; L = nl; break;
+            case DFFT_DHT: op = OP_DHT; L = nl; break;
+            case DFFT_REDFT11: op = OP_DCT4; L = 2 * nl; break;
+            case DFFT_RODFT11: op = OP_DST4; L = 2 * nl; break;
+            case DFFT_REDFT00:
+                if (nl < 2) return DFFT_ERR_ARG;  // FFTW's REDFT00 needs n > 1
+                op = OP_DCT1;
+                L = 2 * (nl - 1);
+                break;
+            case DFFT_RODFT00: op = OP_DST1; L = 2 * (nl + 1); break;
+            default: return DFFT_ERR_UNSUPPORTED;  // R2HC / HC2R: not in the reference's R2R kinds
+            }
+            const double sc = (p->flags & DFFT_FLAG_NORMALIZE_R2R) ? 1.0 / r2r_logical(kinds[i], nl) : 1.0;
+            add(ax, op, L, nl, n, n, i == 0 ? BUF_IN : BUF_OUT, BUF_OUT, sc, false, 2.0 * N * es_r);
         }
     }
 

@@ -17,7 +17,15 @@ enum PassOp : int {
     OP_C2R_FULL = 4,  // L/2+1 bins -> real line of length L (odd) via an L-point inverse FFT
     OP_DCT2 = 5,      // REDFT10 via an L-point FFT of the even/odd reordered line (Makhoul 1980)
     OP_DCT3 = 6,      // REDFT01 via a pre-twiddled L-point inverse FFT (Makhoul 1980)
-    OP_COUNT = 7
+    // the other FFTW r2r kinds; n = logical line length (AxisPass::nlog)
+    OP_DHT = 7,       // DHT: L = n; H = Re F - Im F of the real line's DFT
+    OP_DST2 = 8,      // RODFT10: the DCT-II path on (-1)^j x_j, written in reverse order
+    OP_DST3 = 9,      // RODFT01: the DCT-III path on the reversed line, times (-1)^k
+    OP_DCT1 = 10,     // REDFT00: L = 2(n-1) FFT of the even extension, real part of bins 0..n-1
+    OP_DST1 = 11,     // RODFT00: L = 2(n+1) FFT of the odd extension, -Im of bins 1..n
+    OP_DCT4 = 12,     // REDFT11: L = 2n FFT of x_j e^{-i pi j/2n}; 2 Re(e^{-i pi (2k+1)/4n} F_k)
+    OP_DST4 = 13,     // RODFT11: same FFT; -2 Im(e^{-i pi (2k+1)/4n} F_k)
+    OP_COUNT = 14
 };
 
 // Stockham schedule of one line length.
@@ -44,7 +52,7 @@ struct AxisPass {
     int TC;                   // lines (contig) / columns (strided) per workgroup
     int lds_pitch;            // contig: elements per line slot; strided: elements per row
     int pad_shift;            // contig: slot index p -> p + (p >> pad_shift)
-    int nlog;                 // logical line length (real length for R2C/C2R/DCT)
+    int nlog;                 // logical line length (real length for R2C/C2R/R2R)
     int nt;                   // 1: streaming (nontemporal) global loads/stores
     int variant;              // 0: runtime schedule (any length); 1: compile-time schedule of this
                               // length, 8 values/thread; 2: compile-time schedule, 16 values/thread,

@@ -1,5 +1,6 @@
-"""R2C / C2R (rfft / irfft, README.md:38-44) and R2R DCT-II / DCT-III
-(REDFT10 / REDFT01, README.md:59-66) on the HIP path vs the oracle.
+"""R2C / C2R (rfft / irfft, README.md:38-44) and the R2R kinds (the DCT-II /
+DCT-III pair of README.md:59-66 and every other kind of R2R_SUPPORTED_KINDS,
+src/fft.jl:1-11) on the HIP path vs the oracle.
 FP64 tolerance: relative L2 <= 1e-12."""
 import numpy as np
 import pytest
@@ -121,8 +122,61 @@ def test_mixed_r2r_kinds_per_dim(gpu):
 
 
 def test_unsupported_r2r_kind_raises(gpu):
+    """R2HC / HC2R are FFTW kinds outside the reference's R2R_SUPPORTED_KINDS
+    (src/fft.jl:1-11); REDFT00 needs n > 1 and a 7-smooth 2(n-1)."""
     import daggerfft as D
     from daggerfft import _lib as L
     with pytest.raises(D.DFFTError) as ei:
-        D.Plan((8, 8), L.R2R, -1, L.F64, D.Pencil(), (1, 2), r2r_kinds=(D.DHT, D.DHT))
+        D.Plan((8, 8), L.R2R, -1, L.F64, D.Pencil(), (1, 2), r2r_kinds=(D.R2HC, D.R2HC))
     assert ei.value.status == L.DFFT_ERR_UNSUPPORTED
+    with pytest.raises(D.DFFTError) as ei:
+        D.Plan((1, 8), L.R2R, -1, L.F64, D.Pencil(), (1, 2), r2r_kinds=(D.REDFT00, D.REDFT00))
+    assert ei.value.status == L.DFFT_ERR_ARG
+    with pytest.raises(D.DFFTError) as ei:  # 2 * 127
+        D.Plan((128, 8), L.R2R, -1, L.F64, D.Pencil(), (1, 2), r2r_kinds=(D.REDFT00, D.REDFT00))
+    assert ei.value.status == L.DFFT_ERR_UNSUPPORTED_SIZE
+
+
+# Every kind of the reference's R2R_SUPPORTED_KINDS (src/fft.jl:1-11).  The
+# complex FFT behind each kind has length n (DHT, DCT/DST-II/III), 2n
+# (DCT/DST-IV), 2(n-1) (DCT-I) or 2(n+1) (DST-I); the shapes keep those
+# lengths 7-smooth and include odd n and n = 1.
+R2R_SHAPES = {
+    "REDFT00": [(9, 6, 5), (129, 33, 17), (2, 3, 8)],
+    "RODFT00": [(7, 5, 9), (127, 31, 15), (1, 2, 3)],
+}
+R2R_DEFAULT_SHAPES = [(12, 10, 7), (64, 48, 30), (9, 1, 5), (15, 14, 3)]
+INVERSE = {"REDFT10": "REDFT01", "REDFT01": "REDFT10", "RODFT10": "RODFT01", "RODFT01": "RODFT10"}
+
+
+@pytest.mark.parametrize("kind", ["DHT", "REDFT00", "REDFT01", "REDFT10", "REDFT11", "RODFT00", "RODFT01",
+                                  "RODFT10", "RODFT11"])
+@pytest.mark.parametrize("prec", [1, 0])
+def test_every_r2r_kind_vs_oracle(gpu, kind, prec):
+    """Each kind along every axis (Pencil, all three dims) and along two
+    (Slab), against the oracle's long-double direct sums; then the README
+    inverse (normalised by FFTW's logical size) reconstructs the input.
+    Float64 rel-L2 <= 1e-12; Float32 against the FP64 oracle <= 1e-5 (stated)."""
+    import daggerfft as D
+    from daggerfft import _lib as L
+    k = getattr(D, kind)
+    ki = getattr(D, INVERSE.get(kind, kind))
+    tol = 1e-12 if prec else 1e-5
+    dt = np.float64 if prec else np.float32
+    worst = 0.0
+    for shape in R2R_SHAPES.get(kind, R2R_DEFAULT_SHAPES):
+        a = O.fill(sum(shape), shape, dt)
+        a64 = a.astype(np.float64)
+        for decomp, dims in ((D.Pencil(), (1, 2, 3)), (D.Slab(), (2, 3))):
+            print(kind, "f64" if prec else "f32", shape, dims, flush=True)
+            p = D.Plan(shape, L.R2R, -1, prec, decomp, dims, r2r_kinds=(k,) * len(dims))
+            G = p.execute(_dev(a), D.DeviceArray(shape, dt))
+            ref = O.r2rn(a64, getattr(O, kind), dims)
+            e = O.rel_l2(G.to_host().astype(np.float64), ref)
+            worst = max(worst, e)
+            assert e <= tol, (shape, dims, e)
+            pi = D.Plan(shape, L.R2R, +1, prec, decomp, dims, r2r_kinds=(ki,) * len(dims),
+                        flags=L.FLAG_NORMALIZE_R2R)
+            B = pi.execute(G, D.DeviceArray(shape, dt))
+            assert O.rel_l2(B.to_host().astype(np.float64), a64) <= tol, (shape, dims, "inverse")
+    print(kind, "worst rel-L2 %.3e" % worst)

@@ -97,6 +97,25 @@ def test_dct_vs_scipy(shape):
     assert O.rel_l2(O.r2rn(a, O.REDFT01), sf.dctn(a, type=3)) < 1e-14
 
 
+@pytest.mark.parametrize("kind,ref", [("REDFT00", ("dct", 1)), ("REDFT11", ("dct", 4)), ("RODFT00", ("dst", 1)),
+                                      ("RODFT10", ("dst", 2)), ("RODFT01", ("dst", 3)), ("RODFT11", ("dst", 4))])
+@pytest.mark.parametrize("shape", [(9, 6, 5), (7, 5, 9), (12, 2, 3)])
+def test_r2r_kinds_vs_scipy(kind, ref, shape):
+    """The remaining FFTW kinds against pocketfft's DCT/DST types 1-4
+    (norm=None uses FFTW's unnormalised definitions)."""
+    a = O.fill(16, shape, np.float64)
+    fn = getattr(sf, ref[0] + "n")
+    assert O.rel_l2(O.r2rn(a, getattr(O, kind)), fn(a, type=ref[1])) < 1e-14
+
+
+def test_dht_vs_fft():
+    """DHT: cas kernel = Re - Im of the forward DFT, per dim."""
+    for n in (1, 2, 7, 12):
+        x = O.fill(17, (n,), np.float64)
+        F = np.fft.fft(x)
+        assert O.rel_l2(O.r2rn(x, O.DHT), F.real - F.imag) < 1e-14
+
+
 @pytest.mark.parametrize("dims", [(1,), (2,), (3,), (2, 3), (3, 1, 2)])
 def test_dims_subsets_vs_scipy(dims):
     a = O.fill(3, (12, 10, 6), np.complex128)
