@@ -40,7 +40,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--profile-steps", action="store_true", help="no per-pass events (for rocprof runs)")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
+    ap.add_argument("--no-configs", action="store_true", help="skip the other configurations")
     ap.add_argument("--serve-only", action="store_true", help="only the served-stream measurement")
+    ap.add_argument("--configs-only", action="store_true", help="only the other configurations")
     ap.add_argument("--serve-count", type=int, default=2000, help="requests in the served trace")
     return ap.parse_args()
 
@@ -80,6 +82,118 @@ def cpu_baseline(n):
     return {"value": round(5 * N * math.log2(N) * 1e-9 / t, 3), "unit": "GFLOP/s", "cores": O.num_threads(),
             "kind": "port", "sample": "one full %d^3 ComplexF64 forward + inverse through the C oracle "
             "(per-axis batched FFTs in the reference's stage order), time per transform %.3f s" % (n, t)}
+
+
+def _time_events(fn, reps):
+    """Average ms of fn() over `reps` runs, bracketed by HIP events on torch's stream."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def _pass_table(plan, x, y, reps=5):
+    """Per-axis-pass time and GB/s of one plan (each pass timed on its own)."""
+    out = []
+    for i, d in enumerate(plan.passes()):
+        ms = _time_events(lambda: plan.execute_pass(i, x, y), reps)
+        out.append({"axis": d["axis"] + 1, "op": d["op"], "L": d["L"], "variant": d["variant"], "ms": round(ms, 4),
+                    "GBps": round(d["bytes"] / ms / 1e6, 1)})
+    return out
+
+
+def configs():
+    """The other configurations, reported beside the headline: time, HBM
+    GB/s on the algorithmic bytes (and per axis pass where the pass is the
+    unit), round-trip error, and for the batch the Float32 error against the
+    Float64 GPU result of the same inputs (which equals the oracle to ~1e-15)."""
+    import numpy as np
+    import torch
+
+    import daggerfft as D
+    from daggerfft import _lib as L
+    from daggerfft.batch import Batch, batch_config_sizes
+    out = {}
+    rel = lambda a, b: float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))  # noqa: E731
+
+    # 128^3 ComplexF64 C2C fft + ifft (Pencil)
+    sh = (128, 128, 128)
+    A = D.rand(sh, np.complex128, seed=1)
+    X, Y = A.similar(), A.similar()
+    pf = D.Plan(sh, L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3))
+    pi = D.Plan(sh, L.C2C, +1, L.F64, D.Pencil(), (1, 2, 3))
+    ms = _time_events(lambda: (pf.execute(A, X), pi.execute(X, Y)), 20) / 2
+    out["c2c_128cubed_f64"] = {"ms_per_transform": round(ms, 4), "GBps": round(pf.algorithmic_bytes() / ms / 1e6, 1),
+                               "roundtrip_rel_l2": rel(Y.tensor, A.tensor)}
+    # 256^3 Float64 rfft + irfft, per-pass GB/s, Pencil and Slab
+    sh = (256, 256, 256)
+    A = D.rand(sh, np.float64, seed=2)
+    for name, dec in (("pencil", D.Pencil()), ("slab", D.Slab())):
+        pr = D.Plan(sh, L.R2C, -1, L.F64, dec, (1, 2, 3))
+        pc = D.Plan(sh, L.C2R, +1, L.F64, dec, (1, 2, 3))
+        F = D.DeviceArray((129, 256, 256), np.complex128)
+        B = D.DeviceArray(sh, np.float64)
+        ms_f = _time_events(lambda: pr.execute(A, F), 10)
+        ms_i = _time_events(lambda: pc.execute(F, B), 10)
+        out["r2c_256cubed_f64_" + name] = {
+            "rfft_ms": round(ms_f, 4), "irfft_ms": round(ms_i, 4),
+            "rfft_GBps": round(pr.algorithmic_bytes() / ms_f / 1e6, 1),
+            "irfft_GBps": round(pc.algorithmic_bytes() / ms_i / 1e6, 1),
+            "rfft_passes": _pass_table(pr, A, F) if name == "pencil" else None,
+            "roundtrip_rel_l2": rel(B.tensor, A.tensor)}
+    # DCT-II forward, DCT-III inverse (normalised), Slab
+    for sh, dims in (((256, 256), (1, 2)), ((128, 128, 128), (1, 2, 3))):
+        A = D.rand(sh, np.float64, seed=4 + len(sh))
+        pf = D.Plan(sh, L.R2R, -1, L.F64, D.Slab(), dims, r2r_kinds=(D.REDFT10,) * len(dims))
+        pi = D.Plan(sh, L.R2R, +1, L.F64, D.Slab(), dims, r2r_kinds=(D.REDFT01,) * len(dims),
+                    flags=L.FLAG_NORMALIZE_R2R)
+        F, B = A.similar(), A.similar()
+        ms = _time_events(lambda: (pf.execute(A, F), pi.execute(F, B)), 20) / 2
+        out["dct_%s_f64" % "x".join(map(str, sh))] = {"ms_per_transform": round(ms, 4),
+                                                      "GBps": round(pf.algorithmic_bytes() / ms / 1e6, 1),
+                                                      "roundtrip_rel_l2": rel(B.tensor, A.tensor)}
+    # 384x240x210 ComplexF64 C2C (factors 2, 3, 5, 7), Slab
+    sh = (384, 240, 210)
+    A = D.rand(sh, np.complex128, seed=6)
+    X, Y = A.similar(), A.similar()
+    pf = D.Plan(sh, L.C2C, -1, L.F64, D.Slab(), (1, 2, 3))
+    pi = D.Plan(sh, L.C2C, +1, L.F64, D.Slab(), (1, 2, 3))
+    ms = _time_events(lambda: (pf.execute(A, X), pi.execute(X, Y)), 10) / 2
+    N = float(np.prod(sh))
+    out["c2c_384x240x210_f64"] = {"ms_per_transform": round(ms, 4),
+                                  "GBps": round(pf.algorithmic_bytes() / ms / 1e6, 1),
+                                  "GFLOPs": round(5 * N * math.log2(N) / ms / 1e6, 1),
+                                  "passes": _pass_table(pf, A, X), "roundtrip_rel_l2": rel(Y.tensor, A.tensor)}
+    del A, X, Y
+    # 64 mixed C2C transforms (seed 7): grouped launches, eager and hipGraph replay
+    cfg = batch_config_sizes(64, 7)
+    descs = [(shape, L.C2C, -1, prec) for shape, prec in cfg]
+    ins = [D.rand(shape, np.complex128 if prec else np.complex64, seed=1000 + t) for t, (shape, prec) in enumerate(cfg)]
+    outs = [x.similar() for x in ins]
+    res = {}
+    for name, flags in (("grouped", L.FLAG_BATCH_GROUPED), ("grouped_graph", L.FLAG_BATCH_GROUPED | L.FLAG_GRAPH),
+                        ("lanes", 0), ("lanes_graph", L.FLAG_GRAPH)):
+        b = Batch(descs, flags=flags)
+        ms = _time_events(lambda: b.execute(ins, outs), 10)
+        res[name] = {"ms": round(ms, 4), "GBps": round(b.algorithmic_bytes() / ms / 1e6, 1), "launches": b.launches}
+    # sequential plain calls of the same 64 transforms, for comparison
+    plans = [D.Plan(shape, L.C2C, -1, prec, D.Pencil(), (1, 2, 3)) for shape, prec in cfg]
+    seq = _time_events(lambda: [p.execute(x, y) for p, x, y in zip(plans, ins, outs)], 5)
+    worst32 = 0.0
+    for (shape, prec), x, y in zip(cfg, ins, outs):
+        if prec == L.F32:  # FP32 result against the FP64 transform of the same (widened) input
+            x64 = D.DeviceArray(shape, np.complex128, tensor=x.tensor.to(torch.complex128))
+            worst32 = max(worst32, rel(y.tensor.to(torch.complex128), D.fft(x64).tensor))
+    out["batch64_mixed_c2c"] = {"transforms": 64, **res, "sequential_plans_ms": round(seq, 4),
+                                "f32_rel_l2_vs_f64_max": worst32, "algorithmic_bytes": b.algorithmic_bytes()}
+    torch.cuda.empty_cache()
+    return out
 
 
 def serving(count):
@@ -172,6 +286,9 @@ def main():
     torch.cuda.set_device(0)
     if args.serve_only:
         print(json.dumps({"serving": serving(args.serve_count)}), flush=True)
+        return
+    if args.configs_only:
+        print(json.dumps({"configs": configs()}), flush=True)
         return
     n = args.n
     shape = (n, n, n)
@@ -320,6 +437,11 @@ def main():
     }
     del X, Y, A
     torch.cuda.empty_cache()
+    if not args.no_configs:
+        try:
+            out["configs"] = configs()
+        except Exception as e:  # reported, never fatal
+            out["configs"] = {"error": repr(e)}
     if not args.no_serve:
         try:
             out["serving"] = serving(args.serve_count)

@@ -287,6 +287,21 @@ struct dfft_batch {
     void *d_tw64 = nullptr, *d_tw32 = nullptr;
     size_t tw64_bytes = 0, tw32_bytes = 0;
     double alg_bytes = 0;
+    // lanes (default): every transform runs its own plan (the catalog
+    // kernels) on one of a few streams forked from the caller's stream
+    bool lanes = false;
+    std::vector<dfft_plan *> plans;
+    std::vector<int> order;     // transforms, largest first
+    std::vector<int> lane_of;   // transform -> stream (balanced by bytes)
+    std::vector<hipStream_t> streams;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> joins;
+    bool use_graph = false;
+    std::mutex gmu;             // the lanes recorded as one hipGraph (stream capture)
+    hipStream_t origin = nullptr;
+    hipGraph_t lgraph = nullptr;
+    hipGraphExec_t lexec = nullptr;
+    std::vector<void *> lslots;
 };
 
 static void batch_free(dfft_batch *b) {
@@ -295,7 +310,116 @@ static void batch_free(dfft_batch *b) {
     group_destroy(b->g);
     if (b->d_tw64) (void)hipFree(b->d_tw64);
     if (b->d_tw32) (void)hipFree(b->d_tw32);
+    if (b->lexec) (void)hipGraphExecDestroy(b->lexec);
+    if (b->lgraph) (void)hipGraphDestroy(b->lgraph);
+    for (dfft_plan *p : b->plans) dfft_plan_destroy(p);
+    for (hipStream_t st : b->streams) (void)hipStreamDestroy(st);
+    if (b->origin) (void)hipStreamDestroy(b->origin);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    for (hipEvent_t e : b->joins) (void)hipEventDestroy(e);
     delete b;
+}
+
+namespace dfft {
+int plan_pass_count(const dfft_plan *p);
+PassPlan plan_pass(const dfft_plan *p, int i);
+}
+
+constexpr int BATCH_LANES = 4;
+
+// lanes mode: builds one plan per transform and the streams/events
+static int lanes_create(dfft_batch *b, uint32_t flags) {
+    const int dims[3] = {0, 1, 2};
+    std::vector<double> bytes(b->count);
+    for (int t = 0; t < b->count; ++t) {
+        const dfft_batch_desc &d = b->descs[t];
+        dfft_plan *p = nullptr;
+        const uint32_t pf = (d.flags & DFFT_FLAG_UNNORMALIZED) | (flags & DFFT_FLAG_NO_TUNE);
+        int rc = dfft_plan_create(&p, 3, d.n, d.kind, d.sign, nullptr, d.precision, DFFT_PENCIL, dims, 3, pf);
+        if (rc != DFFT_OK) return rc;
+        b->plans.push_back(p);
+        dfft_plan_algorithmic_bytes(p, &bytes[t]);
+        b->alg_bytes += bytes[t];
+    }
+    b->order.resize(b->count);
+    std::iota(b->order.begin(), b->order.end(), 0);
+    std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) { return bytes[x] > bytes[y]; });
+    const int nl = std::min(BATCH_LANES, b->count);
+    std::vector<double> load(nl, 0.0);
+    b->lane_of.assign(b->count, 0);
+    for (int t : b->order) {
+        const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        b->lane_of[t] = l;
+        load[l] += bytes[t];
+    }
+    for (int l = 0; l < nl; ++l) {
+        hipStream_t st;
+        hipEvent_t ev;
+        if (hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) return hip_fail(e, "hipStreamCreateWithFlags");
+        b->streams.push_back(st);
+        if (hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) return hip_fail(e, "hipEventCreate");
+        b->joins.push_back(ev);
+    }
+    if (hipError_t e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming)) return hip_fail(e, "hipEventCreate");
+    if (b->use_graph) {
+        if (hipError_t e = hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking))
+            return hip_fail(e, "hipStreamCreateWithFlags");
+    }
+    return DFFT_OK;
+}
+
+// lanes mode: fork from `st`, each transform's passes on its stream, join back into `st`
+static int lanes_enqueue(dfft_batch *b, void *const *d_in, void *const *d_out, hipStream_t st) {
+    if (hipError_t e = hipEventRecord(b->fork, st)) return hip_fail(e, "hipEventRecord");
+    for (hipStream_t s : b->streams)
+        if (hipError_t e = hipStreamWaitEvent(s, b->fork, 0)) return hip_fail(e, "hipStreamWaitEvent");
+    for (int t : b->order) {
+        int rc = dfft_execute(b->plans[t], d_in[t], d_out[t], nullptr, b->streams[b->lane_of[t]]);
+        if (rc != DFFT_OK) return rc;
+    }
+    for (size_t l = 0; l < b->streams.size(); ++l) {
+        if (hipError_t e = hipEventRecord(b->joins[l], b->streams[l])) return hip_fail(e, "hipEventRecord");
+        if (hipError_t e = hipStreamWaitEvent(st, b->joins[l], 0)) return hip_fail(e, "hipStreamWaitEvent");
+    }
+    return DFFT_OK;
+}
+
+static int lanes_execute(dfft_batch *b, void *const *d_in, void *const *d_out, hipStream_t st) {
+    if (!b->use_graph) return lanes_enqueue(b, d_in, d_out, st);
+    std::lock_guard<std::mutex> lk(b->gmu);
+    std::vector<void *> cur;
+    for (int t = 0; t < b->count; ++t) {
+        cur.push_back(d_in[t]);
+        cur.push_back(d_out[t]);
+    }
+    if (!b->lexec || cur != b->lslots) {
+        if (b->lexec) (void)hipGraphExecDestroy(b->lexec);
+        if (b->lgraph) (void)hipGraphDestroy(b->lgraph);
+        b->lexec = nullptr;
+        b->lgraph = nullptr;
+        // record the fork/join pattern from a private stream: the lanes join the
+        // capture through the fork event, and the graph keeps their parallel branches
+        if (hipError_t e = hipStreamBeginCapture(b->origin, hipStreamCaptureModeThreadLocal))
+            return hip_fail(e, "hipStreamBeginCapture");
+        int rc = lanes_enqueue(b, d_in, d_out, b->origin);
+        hipGraph_t graph = nullptr;
+        hipError_t ec = hipStreamEndCapture(b->origin, &graph);
+        if (rc != DFFT_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (ec != hipSuccess) return hip_fail(ec, "hipStreamEndCapture");
+        hipGraphExec_t exec;
+        if (hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0)) {
+            (void)hipGraphDestroy(graph);
+            return hip_fail(e, "hipGraphInstantiate");
+        }
+        b->lgraph = graph;
+        b->lexec = exec;
+        b->lslots = cur;
+    }
+    if (hipError_t e = hipGraphLaunch(b->lexec, st)) return hip_fail(e, "hipGraphLaunch");
+    return DFFT_OK;
 }
 
 extern "C" {
@@ -318,6 +442,17 @@ int dfft_batch_create(dfft_batch **batch, int count, const dfft_batch_desc *desc
     if (!b) return DFFT_ERR_OOM;
     b->count = count;
     b->descs.assign(descs, descs + count);
+    if (!(flags & DFFT_FLAG_BATCH_GROUPED)) {
+        b->lanes = true;
+        b->use_graph = (flags & DFFT_FLAG_GRAPH) != 0;
+        int rc = lanes_create(b, flags);
+        if (rc != DFFT_OK) {
+            batch_free(b);
+            return rc;
+        }
+        *batch = b;
+        return DFFT_OK;
+    }
     TwiddleTable tt64, tt32;
     tt64.prec = 1;
     tt32.prec = 0;
@@ -415,19 +550,33 @@ int dfft_batch_execute(dfft_batch *b, void *const *d_in, void *const *d_out, voi
         slots[2 * t] = d_in[t];
         slots[2 * t + 1] = d_out[t];
     }
+    if (b->lanes) return lanes_execute(b, d_in, d_out, (hipStream_t)stream);
     if (b->graph) return group_launch_graph(b->g, b->graph, slots.data(), (int)slots.size(), (hipStream_t)stream);
     return group_launch(b->g, slots.data(), (int)slots.size(), (hipStream_t)stream);
 }
 
 int dfft_batch_num_items(const dfft_batch *b, int64_t *items) {
     if (!b || !items) return DFFT_ERR_ARG;
-    *items = group_blocks(b->g);
+    if (b->lanes) {
+        int64_t n = 0;
+        for (const dfft_plan *p : b->plans)
+            for (int i = 0; i < plan_pass_count(p); ++i) n += plan_pass(p, i).nblocks;
+        *items = n;
+    } else {
+        *items = group_blocks(b->g);
+    }
     return DFFT_OK;
 }
 
 int dfft_batch_num_launches(const dfft_batch *b, int *launches) {
     if (!b || !launches) return DFFT_ERR_ARG;
-    *launches = group_num_launches(b->g);
+    if (b->lanes) {
+        int n = 0;
+        for (const dfft_plan *p : b->plans) n += plan_pass_count(p);
+        *launches = n;
+    } else {
+        *launches = group_num_launches(b->g);
+    }
     return DFFT_OK;
 }
 

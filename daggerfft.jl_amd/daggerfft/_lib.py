@@ -33,6 +33,7 @@ FLAG_NO_TUNE = 0x8
 FLAG_GRAPH = 0x10
 FLAG_STAGE_ORDER = 0x20
 FLAG_PLANE_PIPELINE = 0x40
+FLAG_BATCH_GROUPED = 0x80
 
 _lib = None
 _lock = threading.Lock()

@@ -92,6 +92,8 @@ extern "C" {
                                           contiguous axis runs first -- same result, the passes commute */
 #define DFFT_FLAG_PLANE_PIPELINE 0x40u /* Slab: software-pipeline the 2D stage over z-plane chunks
                                           (grouped launches re-reading each chunk from the Infinity Cache) */
+#define DFFT_FLAG_BATCH_GROUPED  0x80u /* batch: one grouped launch per axis-pass level (three launches)
+                                          instead of each transform's own plan on four forked streams */
 
 typedef struct dfft_plan dfft_plan;
 
@@ -207,11 +209,20 @@ int dfft_tune_entries(int *count);
  * No reference counterpart: many independent transforms per kernel launch
  * (the reference runs one cuFFT plan per chunk, src/fftgpu.jl:118-124).
  * Each transform is rank 3 over dims (0,1,2); C2C (either sign) or R2C
- * (forward, even n[0]).  Launch l runs axis pass l of every transform, so a
- * whole batch takes three launches; with DFFT_FLAG_GRAPH at creation they are
- * recorded once as a hipGraph and replayed by every execute (re-recorded when
- * the buffer pointers change).  Results equal each transform run alone up to
- * rounding (the batch kernel follows the runtime schedule). */
+ * (forward, even n[0]).  Two schedules:
+ *   lanes (default)          each transform's own plan (the compile-time
+ *                            kernels) on one of four streams forked from and
+ *                            joined back into the caller's stream; results
+ *                            equal each transform run alone bit for bit;
+ *   DFFT_FLAG_BATCH_GROUPED  launch l runs axis pass l of every transform, so
+ *                            a whole batch takes three launches (the kernel
+ *                            follows the runtime schedule; results equal each
+ *                            transform alone up to rounding).  Measured 2.9x
+ *                            slower than lanes on the 64-transform mix: every
+ *                            tile runs with the launch's largest block.
+ * With DFFT_FLAG_GRAPH at creation the launches are recorded once as a
+ * hipGraph (the lanes as parallel branches) and replayed by every execute,
+ * re-recorded when the buffer pointers change. */
 typedef struct {
     int64_t n[3];       /* size of this 3D transform (n[0] fastest) */
     int kind;           /* DFFT_C2C or DFFT_R2C */

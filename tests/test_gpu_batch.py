@@ -103,8 +103,9 @@ def test_batch_spin_limit_reports_error(gpu):
 
 def _run_grouped(descs, seeds):
     import daggerfft as D
+    from daggerfft import _lib as L
     from daggerfft.batch import Batch
-    b = Batch(descs)
+    b = Batch(descs, flags=L.FLAG_BATCH_GROUPED)
     assert b.launches == 3
     ins, outs = [], []
     for (shape, kind, sign, prec), sd in zip(descs, seeds):
@@ -175,8 +176,8 @@ def test_grouped_batch_graph_replays(gpu):
     from daggerfft.batch import Batch
     descs = [((32, 48, 20), L.C2C, -1, 1), ((64, 30, 14), L.R2C, -1, 1), ((16, 16, 16), L.C2C, +1, 0),
              ((40, 21, 10), L.R2C, -1, 0), ((96, 32, 48), L.C2C, -1, 1)]
-    eager = Batch(descs)
-    graph = Batch(descs, flags=L.FLAG_GRAPH)
+    eager = Batch(descs, flags=L.FLAG_BATCH_GROUPED)
+    graph = Batch(descs, flags=L.FLAG_BATCH_GROUPED | L.FLAG_GRAPH)
 
     def buffers(seed0):
         ins, oe, og = [], [], []
@@ -214,3 +215,46 @@ def test_grouped_batch_graph_replays(gpu):
         x = X.to_host().astype(np.complex128 if kind == L.C2C else np.float64)
         ref = (O.fftn(x) if sign < 0 else O.fftn(x, inverse=True)) if kind == L.C2C else O.rfftn(x)
         assert O.rel_l2(Y.to_host().astype(np.complex128), ref) <= (1e-12 if prec else 1e-5), t
+
+
+@pytest.mark.parametrize("flags", ["lanes", "lanes_graph"])
+def test_lanes_batch_equals_plans_bitwise(gpu, flags):
+    """The default (lanes) schedule: each transform runs its own plan on one of four
+    forked streams (optionally recorded as one hipGraph with parallel
+    branches); results equal the plain plan of each transform bit for bit,
+    over replays with new data and eager work in between."""
+    import ctypes
+
+    import torch
+
+    import daggerfft as D
+    from daggerfft import _lib as L
+    from daggerfft.batch import Batch, batch_config_sizes
+    cfg = batch_config_sizes(24, 9)
+    descs = [(shape, L.C2C, -1 if t % 3 else +1, prec) for t, (shape, prec) in enumerate(cfg)]
+    descs += [((64, 30, 14), L.R2C, -1, 1), ((40, 21, 10), L.R2C, -1, 0)]
+    fl = L.FLAG_GRAPH if flags == "lanes_graph" else 0
+    b = Batch(descs, flags=fl)
+    assert b.launches == 3 * len(descs)
+    ins, outs = [], []
+    for t, (shape, kind, sign, prec) in enumerate(descs):
+        cplx = kind == L.C2C
+        dt = (np.complex128 if prec else np.complex64) if cplx else (np.float64 if prec else np.float32)
+        ins.append(D.rand(shape, dt, seed=300 + t))
+        oshape = shape if cplx else (shape[0] // 2 + 1,) + tuple(shape[1:])
+        outs.append(D.DeviceArray(oshape, np.complex128 if prec else np.complex64))
+    scratch = torch.rand(1 << 20, device="cuda")
+    for rep in range(3):
+        for t, X in enumerate(ins):
+            L.check(L.lib().dfft_fill_uniform(ctypes.c_void_p(X.ptr), X.size, 1 if X.is_complex else 0,
+                                              X.precision, ctypes.c_uint64(500 + 50 * rep + t),
+                                              D._stream_handle(None)))
+        b.execute(ins, outs)
+        scratch.mul_(1.5).add_(0.25)
+        torch.cuda.synchronize()
+        for t, ((shape, kind, sign, prec), X, Y) in enumerate(zip(descs, ins, outs)):
+            if kind == L.R2C:
+                alone = D.rfft(X)
+            else:
+                alone = D.fft(X) if sign < 0 else D.ifft(X)
+            assert torch.equal(Y.tensor, alone.tensor), (rep, t)
