@@ -380,32 +380,74 @@ struct Radices {
     }
 };
 
-// Compile-time factorization in the runtime planner's greedy order: radix-8
-// stages first, then 4, 2, 3, 5, 7.
+// Compile-time factorizations.
+//   EMAX 8:  the runtime planner's greedy order -- radix-8 stages first, then
+//            4, 2, 3, 5, 7.
+//   EMAX 16: the fewest stages with radices up to 16 (the composite
+//            butterflies 6, 9, 10, 12, 14, 15, 16 included), ties broken by
+//            the fewest threads per line: every stage is one round trip
+//            through LDS, so 486 = 9*9*6 (3 stages) beats 2*3*3*3*3*3 (6).
 struct Factors {
     int S;
     int R[MAX_STAGES];
 };
-constexpr Factors factors_of(int L) {
-    Factors f{0, {0}};
-    constexpr int order[6] = {8, 4, 2, 3, 5, 7};
-    int n = L;
-    for (int i = 0; i < 6; ++i)
-        while (n > 1 && n % order[i] == 0 && f.S < MAX_STAGES) {
-            f.R[f.S++] = order[i];
-            n /= order[i];
-        }
-    if (n != 1) f.S = -1;
-    return f;
+constexpr int stage_tpl(const int *R, int S, int L, int emax) {
+    int t = 1;
+    for (int s = 0; s < S; ++s) {
+        const int per = emax / R[s], need = (L / R[s] + per - 1) / per;
+        if (need > t) t = need;
+    }
+    return t;
 }
-template <int L, class Seq>
+constexpr void min_stage_search(int rem, int start, Factors &cur, int L, int emax, Factors &best, int &best_tpl) {
+    constexpr int set[13] = {16, 15, 14, 12, 10, 9, 8, 7, 6, 5, 4, 3, 2};
+    if (rem == 1) {
+        const int t = stage_tpl(cur.R, cur.S, L, emax);
+        if (best.S < 0 || cur.S < best.S || (cur.S == best.S && t < best_tpl)) {
+            best = cur;
+            best_tpl = t;
+        }
+        return;
+    }
+    if (cur.S >= MAX_STAGES || (best.S >= 0 && cur.S + 1 > best.S)) return;
+    for (int i = start; i < 13; ++i) {
+        const int r = set[i];
+        if (r > emax || rem % r) continue;
+        cur.R[cur.S++] = r;
+        min_stage_search(rem / r, i, cur, L, emax, best, best_tpl);
+        --cur.S;
+    }
+}
+constexpr Factors factors_of(int L, int emax) {
+    Factors f{0, {0}};
+    if (emax < 16) {
+        constexpr int order[6] = {8, 4, 2, 3, 5, 7};
+        int n = L;
+        for (int i = 0; i < 6; ++i)
+            while (n > 1 && n % order[i] == 0 && f.S < MAX_STAGES) {
+                f.R[f.S++] = order[i];
+                n /= order[i];
+            }
+        if (n != 1) f.S = -1;
+        return f;
+    }
+    Factors cur{0, {0}}, best{-1, {0}};
+    int best_tpl = 0;
+    min_stage_search(L, 0, cur, L, emax, best, best_tpl);
+    return best;
+}
+template <int L, int EMAX, class Seq>
 struct RadicesOfImpl;
-template <int L, int... I>
-struct RadicesOfImpl<L, std::integer_sequence<int, I...>> {
-    using type = Radices<factors_of(L).R[I]...>;
+template <int L, int EMAX, int... I>
+struct RadicesOfImpl<L, EMAX, std::integer_sequence<int, I...>> {
+    using type = Radices<factors_of(L, EMAX).R[I]...>;
 };
+template <int L, int EMAX>
+using RadicesOf = typename RadicesOfImpl<L, EMAX, std::make_integer_sequence<int, factors_of(L, EMAX).S>>::type;
 template <int L>
-using RadicesOf = typename RadicesOfImpl<L, std::make_integer_sequence<int, factors_of(L).S>>::type;
+using RadicesOf8 = RadicesOf<L, 8>;  // (single template argument: usable inside macros)
+template <int L>
+using RadicesOf16 = RadicesOf<L, 16>;
 
 // threads per line such that each stage's L/R butterflies fit EMAX/R per thread
 template <class RL, int EMAX>

@@ -49,16 +49,16 @@ struct FixedTable {
 // C2C of length L, 16 values per thread: full (3) and split (2) LDS exchange,
 // contiguous and strided
 #define DFFT_C2C_16(T, P, L)                                           \
-    DFFT_FX(T, P, OP_C2C, 1, 3, 16, false, RadicesOf<L>),             \
-        DFFT_FX(T, P, OP_C2C, 1, 2, 16, true, RadicesOf<L>),          \
-        DFFT_FX(T, P, OP_C2C, 0, 3, 16, false, RadicesOf<L>),         \
-        DFFT_FX(T, P, OP_C2C, 0, 2, 16, true, RadicesOf<L>)
+    DFFT_FX(T, P, OP_C2C, 1, 3, 16, false, RadicesOf16<L>),         \
+        DFFT_FX(T, P, OP_C2C, 1, 2, 16, true, RadicesOf16<L>),      \
+        DFFT_FX(T, P, OP_C2C, 0, 3, 16, false, RadicesOf16<L>),     \
+        DFFT_FX(T, P, OP_C2C, 0, 2, 16, true, RadicesOf16<L>)
 #define DFFT_C2C_F64(L) DFFT_C2C_16(double, 1, L)
 #define DFFT_C2C_F32(L) DFFT_C2C_16(float, 0, L)
 // R2C of a real line of 2H points through an H-point FFT (contiguous axis)
 #define DFFT_R2C(H)                                                    \
-    DFFT_FX(double, 1, OP_R2C, 1, 3, 16, false, RadicesOf<H>),        \
-        DFFT_FX(float, 0, OP_R2C, 1, 3, 16, false, RadicesOf<H>)
+    DFFT_FX(double, 1, OP_R2C, 1, 3, 16, false, RadicesOf16<H>),    \
+        DFFT_FX(float, 0, OP_R2C, 1, 3, 16, false, RadicesOf16<H>)
 
 #define DFFT_TABLE(name, ...)                                           \
     static const FixedEntry name##_e[] = {__VA_ARGS__};                 \

@@ -5,11 +5,12 @@
 
 namespace dfft {
 
+// the radix-8 lists of both variants (8*8*8 for 512, ...) are the measured ones
 #define DFFT_POW2(T, P, L)                                              \
-    DFFT_FX(T, P, OP_C2C, 1, 1, 8, false, RadicesOf<L>),               \
-        DFFT_FX(T, P, OP_C2C, 1, 2, 16, true, RadicesOf<L>),           \
-        DFFT_FX(T, P, OP_C2C, 0, 1, 8, false, RadicesOf<L>),           \
-        DFFT_FX(T, P, OP_C2C, 0, 2, 16, true, RadicesOf<L>)
+    DFFT_FX(T, P, OP_C2C, 1, 1, 8, false, RadicesOf8<L>),            \
+        DFFT_FX(T, P, OP_C2C, 1, 2, 16, true, RadicesOf8<L>),        \
+        DFFT_FX(T, P, OP_C2C, 0, 1, 8, false, RadicesOf8<L>),        \
+        DFFT_FX(T, P, OP_C2C, 0, 2, 16, true, RadicesOf8<L>)
 
 DFFT_TABLE(g_fixed_pow2,
     DFFT_POW2(double, 1, 64), DFFT_POW2(double, 1, 128), DFFT_POW2(double, 1, 256),
