@@ -8,8 +8,8 @@
 // Variants (AxisPass::variant):
 //   1  8 values per thread, whole complex values exchanged through LDS
 //   2  16 values per thread, real and imaginary parts exchanged in two
-//      rounds through half the LDS (C2C only: the fused R2C/C2R steps keep
-//      whole complex values in LDS)
+//      rounds through half the LDS (C2C and DCT-II/III; the fused R2C/C2R
+//      steps keep whole complex values in LDS)
 //   3  16 values per thread, whole complex values exchanged through LDS
 //      (the mixed-radix lengths: radix-3/5/7 stages leave at most 1-2 of
 //      the 16 registers idle, against 3 of 8)
@@ -59,6 +59,15 @@ struct FixedTable {
 #define DFFT_R2C(H)                                                    \
     DFFT_FX(double, 1, OP_R2C, 1, 3, 16, false, RadicesOf16<H>),    \
         DFFT_FX(float, 0, OP_R2C, 1, 3, 16, false, RadicesOf16<H>)
+
+// DCT-II / DCT-III (REDFT10 / REDFT01, Makhoul's n-point algorithm) of
+// length L, split LDS exchange, both layouts: their fused load and store maps
+// touch only global memory, so the split exchange applies
+#define DFFT_DCT(T, P, L)                                              \
+    DFFT_FX(T, P, OP_DCT2, 1, 2, 16, true, RadicesOf16<L>),           \
+        DFFT_FX(T, P, OP_DCT2, 0, 2, 16, true, RadicesOf16<L>),       \
+        DFFT_FX(T, P, OP_DCT3, 1, 2, 16, true, RadicesOf16<L>),       \
+        DFFT_FX(T, P, OP_DCT3, 0, 2, 16, true, RadicesOf16<L>)
 
 #define DFFT_TABLE(name, ...)                                           \
     static const FixedEntry name##_e[] = {__VA_ARGS__};                 \

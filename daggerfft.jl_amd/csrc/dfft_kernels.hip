@@ -12,6 +12,7 @@
 #include "dfft_kernels.hpp"
 #include "dfft_plan.hpp"
 
+#include <algorithm>
 #include <vector>
 
 namespace dfft {
@@ -45,34 +46,52 @@ static axis_kernel_t g_axis_kernels[2][2][OP_COUNT] = {
 // compile-time schedules (the catalog tables live in dfft_fixed_*.hip)
 // ---------------------------------------------------------------------------
 extern const FixedTable g_fixed_pow2, g_fixed_f64a, g_fixed_f64b, g_fixed_f64c, g_fixed_f32a, g_fixed_f32b,
-    g_fixed_f32c, g_fixed_r2ca, g_fixed_r2cb;
+    g_fixed_f32c, g_fixed_r2ca, g_fixed_r2cb, g_fixed_dct;
 static const FixedTable *const g_fixed_tables[] = {&g_fixed_pow2, &g_fixed_f64a, &g_fixed_f64b, &g_fixed_f64c,
                                                    &g_fixed_f32a, &g_fixed_f32b, &g_fixed_f32c, &g_fixed_r2ca,
-                                                   &g_fixed_r2cb};
+                                                   &g_fixed_r2cb, &g_fixed_dct};
 
-static const FixedEntry *fixed_entry(int prec, int contig, int variant, int op, int L) {
+// The catalog kernel of (precision, layout, variant, op, length) with radix
+// list R[0..S) -- or, with R == nullptr, the first one listed (the default).
+static const FixedEntry *fixed_entry(int prec, int contig, int variant, int op, int L, const int *R, int S) {
     if (variant < 1) return nullptr;
     for (const FixedTable *t : g_fixed_tables)
         for (int i = 0; i < t->n; ++i) {
             const FixedEntry &e = t->e[i];
-            if (e.prec == (prec ? 1 : 0) && e.op == op && e.contig == (contig ? 1 : 0) && e.variant == variant &&
-                e.L == L)
-                return &e;
+            if (e.prec != (prec ? 1 : 0) || e.op != op || e.contig != (contig ? 1 : 0) || e.variant != variant ||
+                e.L != L)
+                continue;
+            if (R && (e.S != S || !std::equal(R, R + S, e.R))) continue;
+            return &e;
         }
     return nullptr;
 }
 
 bool fixed_schedule(int prec, int contig, int variant, int op, int L, std::vector<int> &radices, int &tpl) {
-    const FixedEntry *e = fixed_entry(prec, contig, variant, op, L);
+    const FixedEntry *e = radices.empty() ? fixed_entry(prec, contig, variant, op, L, nullptr, 0)
+                                          : fixed_entry(prec, contig, variant, op, L, radices.data(),
+                                                        (int)radices.size());
     if (!e) return false;
     radices.assign(e->R, e->R + e->S);
     tpl = e->tpl;
     return true;
 }
 
+std::vector<std::vector<int>> fixed_lists(int prec, int contig, int variant, int op, int L) {
+    std::vector<std::vector<int>> out;
+    for (const FixedTable *t : g_fixed_tables)
+        for (int i = 0; i < t->n; ++i) {
+            const FixedEntry &e = t->e[i];
+            if (e.prec == (prec ? 1 : 0) && e.op == op && e.contig == (contig ? 1 : 0) && e.variant == variant &&
+                e.L == L)
+                out.emplace_back(e.R, e.R + e.S);
+        }
+    return out;
+}
+
 static axis_kernel_t axis_kernel(const AxisPass &a, int prec) {
     if (a.variant >= 1) {
-        const FixedEntry *e = fixed_entry(prec, a.contig, a.variant, a.op, a.sp.L);
+        const FixedEntry *e = fixed_entry(prec, a.contig, a.variant, a.op, a.sp.L, a.sp.R, a.sp.S);
         return e ? e->fn : nullptr;
     }
     return g_axis_kernels[prec ? 1 : 0][a.contig ? 1 : 0][a.op];

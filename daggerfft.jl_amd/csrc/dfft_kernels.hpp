@@ -12,8 +12,12 @@ namespace dfft {
 int axis_lds_limit();
 hipError_t launch_axis(const AxisPass &a, int prec, int64_t nblocks, int threads, size_t lds, hipStream_t stream);
 void *axis_kernel_fn(const AxisPass &a, int prec);
-// compile-time schedule of (precision, layout, variant, op, length), if one is built
+// compile-time schedule of (precision, layout, variant, op, length), if one is
+// built: `radices` in = the wanted radix list (empty: the default one), out =
+// the kernel's list
 bool fixed_schedule(int prec, int contig, int variant, int op, int L, std::vector<int> &radices, int &tpl);
+// every radix list the catalog holds for (precision, layout, variant, op, length)
+std::vector<std::vector<int>> fixed_lists(int prec, int contig, int variant, int op, int L);
 hipError_t launch_fill(void *out, int64_t nscalar, int prec, uint64_t seed, hipStream_t stream);
 hipError_t launch_box(const void *src, int64_t s_off, int64_t s_s1, int64_t s_s2, void *dst, int64_t d_off,
                       int64_t d_s1, int64_t d_s2, const int64_t *cnt, int esize, hipStream_t stream);

@@ -186,13 +186,18 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
     a.variant = 0;
     for (int want : prefs) {
         if (want < 1 || (grouped && want != 1)) continue;
+        // a tuned compile-time schedule names its radix list (several per length)
+        R.clear();
+        if (tune && want == tune->variant) R = tune->radices;
         if (fixed_schedule(pp.prec, a.contig, want, a.op, L, R, fixed_tpl)) {
             a.variant = want;
             break;
         }
     }
     if (!a.variant) {
-        if (tune && !tune->radices.empty()) R = tune->radices;
+        // a tuned radix list for the runtime engine (radices up to 8); a
+        // catalog kernel's list (variant >= 1) may hold composite radices
+        if (tune && tune->variant == 0 && !tune->radices.empty()) R = tune->radices;
         else if (!factorize(L, R)) return DFFT_ERR_UNSUPPORTED_SIZE;
     }
     int prod = 1;
@@ -203,8 +208,9 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
     const int emax = a.variant >= 2 ? 16 : EMAX;
     const int lds_es = a.variant == 2 ? es / 2 : es;
     int tpl = a.variant ? fixed_tpl
-                        : (tune && tune->tpl > 0 && tpl_ok(L, R, emax, tune->tpl)) ? tune->tpl
-                                                                                    : default_tpl(L, R, emax, emax);
+              : (tune && tune->variant == 0 && tune->tpl > 0 && tpl_ok(L, R, emax, tune->tpl))
+                  ? tune->tpl
+                  : default_tpl(L, R, emax, emax);
     if (L == 1) tpl = 1;
     if (tpl > max_threads) return DFFT_ERR_UNSUPPORTED_SIZE;
     a.TPL = tpl;

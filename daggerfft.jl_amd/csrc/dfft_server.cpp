@@ -44,10 +44,17 @@ namespace {
 
 using clk = std::chrono::steady_clock;
 
-constexpr size_t GRAIN = 2u << 20;           // allocation granularity (2 MiB)
-constexpr size_t CHUNK = (size_t)8 << 30;     // arena growth step (8 GiB)
+// Buffers of at least LARGE bytes come from large chunks at 32 MiB granularity,
+// smaller ones from small chunks at 2 MiB granularity: small buffers never
+// split the holes that large ones need.
+constexpr size_t LARGE = (size_t)256 << 20;
+constexpr size_t GRAIN_S = (size_t)2 << 20, GRAIN_L = (size_t)32 << 20;
+constexpr size_t CHUNK_S = (size_t)2 << 30, CHUNK_L = (size_t)8 << 30;
 
-size_t round_up(size_t b) { return (b + GRAIN - 1) / GRAIN * GRAIN; }
+size_t round_up(size_t b) {
+    const size_t g = b >= LARGE ? GRAIN_L : GRAIN_S;
+    return (b + g - 1) / g * g;
+}
 
 struct Buf {
     void *p = nullptr;
@@ -64,6 +71,7 @@ struct Arena {
     struct Chunk {
         char *base;
         size_t size;
+        bool large;
         std::map<size_t, size_t> free;  // offset -> length of each free range
     };
     size_t budget = 0;
@@ -72,16 +80,18 @@ struct Arena {
     int64_t allocs = 0, grows = 0, waits = 0;
     std::vector<Chunk> chunks;
 
-    bool carve(size_t bytes, Buf &b) {
+    bool carve(size_t bytes, bool large, Buf &b) {
         int bc = -1;
         size_t boff = 0, blen = ~(size_t)0;
-        for (size_t c = 0; c < chunks.size(); ++c)
+        for (size_t c = 0; c < chunks.size(); ++c) {
+            if (chunks[c].large != large) continue;
             for (const auto &fr : chunks[c].free)
                 if (fr.second >= bytes && fr.second < blen) {
                     bc = (int)c;
                     boff = fr.first;
                     blen = fr.second;
                 }
+        }
         if (bc < 0) return false;
         auto &fl = chunks[bc].free;
         fl.erase(boff);
@@ -98,9 +108,11 @@ struct Arena {
         b = Buf{};
         if (bytes == 0) return DFFT_OK;
         bytes = round_up(bytes);
-        if (carve(bytes, b)) return DFFT_OK;
+        const bool large = bytes >= LARGE;
+        if (carve(bytes, large, b)) return DFFT_OK;
+        const size_t g = large ? GRAIN_L : GRAIN_S;
         const size_t room = budget > held + other ? budget - held - other : 0;
-        size_t want = std::min(room, std::max(bytes, CHUNK)) / GRAIN * GRAIN;
+        size_t want = std::min(room, std::max(bytes, large ? CHUNK_L : CHUNK_S)) / g * g;
         if (want < bytes) return DFFT_ERR_BUDGET;
         void *p = nullptr;
         if (hipMalloc(&p, want) != hipSuccess) {
@@ -111,10 +123,10 @@ struct Arena {
                 return DFFT_ERR_OOM;
             }
         }
-        chunks.push_back(Chunk{static_cast<char *>(p), want, {{0, want}}});
+        chunks.push_back(Chunk{static_cast<char *>(p), want, large, {{0, want}}});
         held += want;
         ++grows;
-        return carve(bytes, b) ? DFFT_OK : DFFT_ERR_OOM;
+        return carve(bytes, large, b) ? DFFT_OK : DFFT_ERR_OOM;
     }
     void release(Buf &b) {
         if (!b.p) return;
@@ -162,6 +174,7 @@ struct Req {
     const void *result = nullptr;
     size_t result_bytes = 0;
     double cost = 0;  // algorithmic bytes
+    bool waited = false;  // admission once deferred for want of a contiguous range
     hipEvent_t ev = nullptr;
     size_t need() const { return cls_in + cls_out + cls_work; }
 };
@@ -323,7 +336,8 @@ bool dfft_server::admit() {
             else break;
         }
         if (rc != DFFT_OK) {
-            ++pool.waits;
+            if (!q->waited) ++pool.waits;
+            q->waited = true;
             if (rc == DFFT_ERR_OOM && running.empty() && pool.busy == 0) {
                 // nothing else holds device memory: it can never fit
                 it = queue.erase(it);

@@ -313,8 +313,9 @@ typedef struct {
     int64_t plans;          /* resident plans */
     int64_t overtakes;      /* admissions that overtook a request waiting for memory */
     int64_t buffer_allocs;  /* request buffers carved from the arena */
-    int64_t arena_chunks;   /* device allocations of the arena (it grows in 8 GiB chunks) */
-    int64_t arena_waits;    /* admissions deferred for want of a contiguous range */
+    int64_t arena_chunks;   /* device allocations of the arena (2 GiB chunks for buffers under
+                               256 MiB, 8 GiB chunks for larger ones) */
+    int64_t arena_waits;    /* requests whose admission waited for a contiguous range */
     size_t budget;
     size_t bytes_held;      /* arena chunks + plans */
     size_t bytes_busy;      /* buffers of running requests and of unreleased results */

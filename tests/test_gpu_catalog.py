@@ -106,3 +106,53 @@ def test_catalog_f64_strided_both_schedules(gpu, tmp_path):
                 assert e <= 1e-12, (n, variant, e)
             finally:
                 D.tune_clear()
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+def test_catalog_pow2_alternative_radix_lists(gpu, tmp_path, prec):
+    """Powers of two have two split-LDS kernels per length (radix-8 lists and
+    fewest-stage lists such as 16*16 for 256); the schedule cache names the
+    list.  Each is forced for both layouts and checked against the oracle."""
+    import daggerfft as D
+    from daggerfft import _lib as L
+    lists = {64: "16,4", 128: "16,8", 256: "16,16", 512: "16,16,2", 1024: "16,16,4"}
+    dt = np.complex128 if prec else np.complex64
+    for n, rad in lists.items():
+        for shape, ax, layout in (((n, 4, 3), 0, 1), ((3, n, 4), 1, 2)):
+            sc = int(math.floor(math.log2(int(np.prod(shape)))))
+            path = os.path.join(str(tmp_path), "tune.txt")
+            with open(path, "w") as f:
+                f.write("%d/0/%d/%d/%d %s 0 %d 0 2\n" % (prec, n, layout, sc, rad, 1 if layout == 1 else 4))
+            D.tune_clear()
+            D.tune_load(path)
+            try:
+                p = D.Plan(shape, L.C2C, -1, prec, D.Pencil(), (1, 2, 3))
+                q = [x for x in p.passes() if x["axis"] == ax][0]
+                assert q["variant"] == 2 and q["radices"] == rad.replace(",", "x"), q
+                a = O.fill(n + ax, shape, dt)
+                F = p.execute(D.DeviceArray.from_host(a), D.DeviceArray(shape, dt))
+                e = O.rel_l2(F.to_host().astype(np.complex128), O.fftn(a.astype(np.complex128)))
+                assert e <= TOL[prec], (n, ax, e)
+            finally:
+                D.tune_clear()
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+def test_catalog_dct_pow2(gpu, prec):
+    """DCT-II / DCT-III catalog kernels (split LDS) of the powers of two
+    16..1024, along the contiguous and a strided axis, against the oracle."""
+    import daggerfft as D
+    from daggerfft import _lib as L
+    D.tune_clear()
+    dt = np.float64 if prec else np.float32
+    tol = 1e-12 if prec else 1e-5
+    for n in (16, 32, 64, 128, 256, 512, 1024):
+        for shape, ax in (((n, 3, 2), 0), ((2, n, 3), 1)):
+            a = O.fill(n + ax, shape, dt)
+            for kind, okind in ((D.REDFT10, O.REDFT10), (D.REDFT01, O.REDFT01)):
+                p = D.Plan(shape, L.R2R, -1, prec, D.Pencil(), (ax + 1,), r2r_kinds=(kind,))
+                q = p.passes()[0]
+                assert q["variant"] == 2 and q["op"] in ("DCT2", "DCT3"), q
+                G = p.execute(D.DeviceArray.from_host(a), D.DeviceArray(shape, dt))
+                e = O.rel_l2(G.to_host().astype(np.float64), O.r2rn(a.astype(np.float64), okind, (ax + 1,)))
+                assert e <= tol, (n, ax, kind, e)
