@@ -504,6 +504,12 @@ __device__ __forceinline__ void run_fixed_stages(const TileCtx<T, OP, CONTIG> &c
 // with TPLC threads per line; otherwise they follow the runtime schedule a.sp.
 struct NoRadices {};
 
+// the tile a single-pass launch's workgroup runs (AxisPass::swz)
+__device__ __forceinline__ int64_t pass_tile(const AxisPass &a) {
+    const uint64_t b = blockIdx.x;
+    return a.swz > 1 ? (int64_t)((b * (uint64_t)a.swz) % (uint64_t)gridDim.x) : (int64_t)b;
+}
+
 template <int EMAX, class T, int OP, bool CONTIG, bool SPLIT = false, class RL = NoRadices, int TPLC = 0>
 __device__ __forceinline__ void axis_tile(const AxisPass &a, const void *in, void *out, int64_t tile, int tid,
                                           cx<T> *lds) {

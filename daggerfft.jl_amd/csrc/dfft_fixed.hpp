@@ -27,7 +27,7 @@ using axis_kernel_t = void (*)(AxisPass);
 template <class T, int OP, bool CONTIG, int EMAX, bool SPLIT, class RL, int TPLC>
 __global__ __launch_bounds__(1024) void k_axis_fixed(AxisPass a) {
     extern __shared__ __align__(16) unsigned char dfft_smem[];
-    axis_tile<EMAX, T, OP, CONTIG, SPLIT, RL, TPLC>(a, a.in, a.out, (int64_t)blockIdx.x, (int)threadIdx.x,
+    axis_tile<EMAX, T, OP, CONTIG, SPLIT, RL, TPLC>(a, a.in, a.out, pass_tile(a), (int)threadIdx.x,
                                                      reinterpret_cast<cx<T> *>(dfft_smem));
 }
 

@@ -23,7 +23,7 @@ namespace dfft {
 template <class T, int OP, bool CONTIG, int EMAX>
 __global__ __launch_bounds__(1024) void k_axis(AxisPass a) {
     extern __shared__ __align__(16) unsigned char dfft_smem[];
-    axis_tile<EMAX, T, OP, CONTIG>(a, a.in, a.out, (int64_t)blockIdx.x, (int)threadIdx.x,
+    axis_tile<EMAX, T, OP, CONTIG>(a, a.in, a.out, pass_tile(a), (int)threadIdx.x,
                                    reinterpret_cast<cx<T> *>(dfft_smem));
 }
 

@@ -17,6 +17,7 @@
 #include <map>
 #include <mutex>
 #include <new>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -248,6 +249,8 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
         pp.nblocks = ((a.nlines + tc - 1) / tc) * a.nouter;
     }
     if (pp.nblocks > 0x7fffffffLL) return DFFT_ERR_UNSUPPORTED_SIZE;
+    a.swz = 1;
+    if (tune && tune->swz > 1 && std::gcd<int64_t, int64_t>(tune->swz, pp.nblocks) == 1) a.swz = tune->swz;
     return DFFT_OK;
 }
 
@@ -808,9 +811,9 @@ int dfft_plan_describe(const dfft_plan *p, int pass, char *buf, size_t buflen) {
     snprintf(buf, buflen,
              "{\"pass\":%d,\"axis\":%d,\"op\":\"%s\",\"L\":%d,\"nlog\":%d,\"radices\":\"%s\",\"TPL\":%d,\"TC\":%d,"
              "\"contig\":%d,\"variant\":%d,\"threads\":%d,\"blocks\":%lld,\"lds\":%zu,\"bytes\":%.0f,\"prec\":\"%s\","
-             "\"kernel\":\"%s\"}",
+             "\"swz\":%d,\"kernel\":\"%s\"}",
              pass, pp.axis, op_name(a.op), a.sp.L, a.nlog, rad.c_str(), a.TPL, a.TC, a.contig, a.variant, pp.threads,
-             (long long)pp.nblocks, pp.lds, pp.alg_bytes, pp.prec ? "f64" : "f32", kname.c_str());
+             (long long)pp.nblocks, pp.lds, pp.alg_bytes, pp.prec ? "f64" : "f32", a.swz, kname.c_str());
     return DFFT_OK;
 }
 

@@ -34,6 +34,7 @@ struct Tuning {
     int tc = 0;
     int pad_shift = 0;
     int variant = 0;  // see AxisPass::variant
+    int swz = 1;      // see AxisPass::swz
 };
 
 // Records `what` and HIP's message as this thread's last error (dfft_last_error)

@@ -136,6 +136,9 @@ static std::vector<Tuning> candidates(const PassPlan &pp, int level, int max_thr
     std::vector<int> tcs = a.contig ? std::vector<int>{1, 2, 4, 8, 16} : std::vector<int>{4, 8, 16, 32, 64};
     std::vector<int> pads = a.contig ? (level >= 2 ? std::vector<int>{3, 4, 5} : std::vector<int>{3, 4})
                                      : std::vector<int>{0};
+    // strided passes also try spreading the tiles that run at one time over
+    // the array (AxisPass::swz; odd multipliers, coprime to power-of-two tile counts)
+    std::vector<int> swizzles = a.contig ? std::vector<int>{1} : std::vector<int>{1, 97, 1021};
     // compile-time schedules of this length: every radix list the catalog
     // holds, times the tile shape
     for (int var : {1, 2, 3}) {
@@ -145,14 +148,16 @@ static std::vector<Tuning> candidates(const PassPlan &pp, int level, int max_thr
             if (!fixed_schedule(pp.prec, a.contig, var, a.op, a.sp.L, R, tpl)) continue;
             for (int tc : tcs) {
                 if (tpl * tc > max_threads) continue;
-                for (int ps : pads) {
-                    Tuning t;
-                    t.variant = var;
-                    t.radices = R;
-                    t.tc = tc;
-                    t.pad_shift = ps;
-                    out.push_back(t);
-                }
+                for (int ps : pads)
+                    for (int sw : swizzles) {
+                        Tuning t;
+                        t.variant = var;
+                        t.radices = R;
+                        t.tc = tc;
+                        t.pad_shift = ps;
+                        t.swz = sw;
+                        out.push_back(t);
+                    }
             }
         }
     }
@@ -273,9 +278,10 @@ int dfft_autotune(int rank, const int64_t *n, int kind, int sign, const int *r2r
             if (verbose) {
                 std::string rad;
                 for (int r : cand.radices) rad += (rad.empty() ? "" : "x") + std::to_string(r);
-                fprintf(stderr, "tune %s radices=%s var=%d tpl=%d tc=%d pad=%d threads=%d lds=%zu: %.4f ms %.1f GB/s\n",
-                        key.c_str(), rad.c_str(), pp.ap.variant, pp.ap.TPL, pp.ap.TC, pp.ap.pad_shift, pp.threads,
-                        pp.lds, ms, pp.alg_bytes / (ms * 1e-3) / 1e9);
+                fprintf(stderr,
+                        "tune %s radices=%s var=%d tpl=%d tc=%d pad=%d swz=%d threads=%d lds=%zu: %.4f ms %.1f GB/s\n",
+                        key.c_str(), rad.c_str(), pp.ap.variant, pp.ap.TPL, pp.ap.TC, pp.ap.pad_shift, pp.ap.swz,
+                        pp.threads, pp.lds, ms, pp.alg_bytes / (ms * 1e-3) / 1e9);
             }
         }
         if (quick.empty()) continue;
@@ -310,12 +316,13 @@ int dfft_tune_save(const char *path) {
     std::ofstream f(path);
     if (!f) return DFFT_ERR_ARG;
     std::lock_guard<std::mutex> g(g_mu);
-    f << "# daggerfft_hip schedule cache: key radices tpl tc pad_shift variant\n";
+    f << "# daggerfft_hip schedule cache: key radices tpl tc pad_shift variant swizzle\n";
     for (const auto &kv : g_cache) {
         f << kv.first << " ";
         for (size_t i = 0; i < kv.second.radices.size(); ++i) f << (i ? "," : "") << kv.second.radices[i];
         if (kv.second.radices.empty()) f << "-";
         f << " " << kv.second.tpl << " " << kv.second.tc << " " << kv.second.pad_shift << " " << kv.second.variant
+          << " " << kv.second.swz
           << "\n";
     }
     return f.good() ? DFFT_OK : DFFT_ERR_ARG;
@@ -340,7 +347,8 @@ static int load_file(const char *path) {
         Tuning t;
         if (!(ss >> key >> rad >> t.tpl >> t.tc >> t.pad_shift)) return DFFT_ERR_ARG;
         if (!(ss >> t.variant)) t.variant = 0;
-        if (t.variant < 0 || t.variant > 3) return DFFT_ERR_ARG;
+        if (!(ss >> t.swz)) t.swz = 1;
+        if (t.variant < 0 || t.variant > 3 || t.swz < 1) return DFFT_ERR_ARG;
         if (rad != "-") {
             std::stringstream rs(rad);
             std::string tok;

@@ -58,6 +58,10 @@ struct AxisPass {
                               // length, 8 values/thread; 2: compile-time schedule, 16 values/thread,
                               // real and imaginary parts exchanged in two rounds (half the LDS);
                               // 3: compile-time schedule, 16 values/thread
+    int swz;                  // workgroup b runs tile (b * swz) mod (number of tiles); 1 = in order.
+                              // Coprime to the tile count, so every tile runs once; spreads the
+                              // tiles running at one time over the array (HBM channel balance
+                              // for lines with a large power-of-two stride)
     int64_t nlines;           // contig: number of lines; strided: number of columns
     int64_t nouter;           // strided: number of outer slices (1 for contig)
     int64_t in_sm, in_so;     // input: stride along the line, stride between lines/outer

@@ -156,3 +156,31 @@ def test_catalog_dct_pow2(gpu, prec):
                 G = p.execute(D.DeviceArray.from_host(a), D.DeviceArray(shape, dt))
                 e = O.rel_l2(G.to_host().astype(np.float64), O.r2rn(a.astype(np.float64), okind, (ax + 1,)))
                 assert e <= tol, (n, ax, kind, e)
+
+
+@pytest.mark.parametrize("swz", [97, 1021])
+def test_swizzled_tile_order(gpu, tmp_path, swz):
+    """A strided pass whose workgroups take the tiles in swizzled order
+    (tile = block * swz mod tiles, forced through the schedule cache) gives
+    the same result as the in-order pass and the oracle."""
+    import daggerfft as D
+    from daggerfft import _lib as L
+    shape = (64, 48, 128)
+    a = O.fill(31, shape, np.complex128)
+    sc = int(math.floor(math.log2(int(np.prod(shape)))))
+    path = os.path.join(str(tmp_path), "tune.txt")
+    with open(path, "w") as f:
+        f.write("1/0/128/0/%d - 0 8 0 2 %d\n" % (sc, swz))   # axis 3: plane layout
+        f.write("1/0/48/2/%d - 0 8 0 2 %d\n" % (sc, swz))    # axis 2: outer slices
+    D.tune_clear()
+    D.tune_load(path)
+    try:
+        p = D.Plan(shape, L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3))
+        sw = {q["axis"]: q["swz"] for q in p.passes()}
+        assert sw[2] == swz and sw[1] == swz and sw[0] == 1, sw
+        F = p.execute(D.DeviceArray.from_host(a), D.DeviceArray(shape, np.complex128))
+    finally:
+        D.tune_clear()
+    ref = D.fft(D.DeviceArray.from_host(a)).to_host()
+    assert np.array_equal(F.to_host(), ref)
+    assert O.rel_l2(F.to_host(), O.fftn(a)) <= 1e-12
