@@ -59,6 +59,11 @@ struct FixedTable {
 #define DFFT_R2C(H)                                                    \
     DFFT_FX(double, 1, OP_R2C, 1, 3, 16, false, RadicesOf16<H>),    \
         DFFT_FX(float, 0, OP_R2C, 1, 3, 16, false, RadicesOf16<H>)
+// C2R of H+1 bins to a real line of 2H points (contiguous axis); the bins
+// are staged in LDS as whole complex values, so the full exchange (3)
+#define DFFT_C2R(H)                                                    \
+    DFFT_FX(double, 1, OP_C2R, 1, 3, 16, false, RadicesOf16<H>),    \
+        DFFT_FX(float, 0, OP_C2R, 1, 3, 16, false, RadicesOf16<H>)
 
 // DCT-II / DCT-III (REDFT10 / REDFT01, Makhoul's n-point algorithm) of
 // length L, split LDS exchange, both layouts: their fused load and store maps

@@ -5,7 +5,7 @@ saves the fastest schedule of every axis pass.
 
   python daggerfft.jl_amd/wisdom/make_wisdom.py [out.txt] [--quick]
 
-Shapes: 512^3 ComplexF64 (full search), 128^3 ComplexF64, 256^3 R2C Float64,
+Shapes: 512^3 ComplexF64 (full search), 128^3 ComplexF64, 256^3 R2C and C2R Float64,
 384x240x210 ComplexF64, the DCT-II/III configurations, and every distinct
 request shape of the mixed serving trace (cubic 64..512, C2C and R2C, both
 precisions).  Progress is printed per shape.
@@ -29,7 +29,8 @@ def main():
     torch.cuda.init()
     D.tune_clear()
     jobs = [((512, 512, 512), L.C2C, L.F64, None, 2), ((128, 128, 128), L.C2C, L.F64, None, 1),
-            ((256, 256, 256), L.R2C, L.F64, None, 1), ((384, 240, 210), L.C2C, L.F64, None, 1),
+            ((256, 256, 256), L.R2C, L.F64, None, 1), ((256, 256, 256), L.C2R, L.F64, None, 1),
+            ((384, 240, 210), L.C2C, L.F64, None, 1),
             ((256, 256), L.R2R, L.F64, (D.REDFT10, D.REDFT10), 1),
             ((128, 128, 128), L.R2R, L.F64, (D.REDFT10,) * 3, 1)]
     seen = set()
@@ -42,7 +43,7 @@ def main():
         jobs = jobs[:8]
     t0 = time.time()
     for i, (shape, kind, prec, r2r, level) in enumerate(jobs):
-        D.autotune(shape, kind=kind, sign=-1, precision=prec, r2r_kinds=r2r, level=level)
+        D.autotune(shape, kind=kind, sign=+1 if kind == L.C2R else -1, precision=prec, r2r_kinds=r2r, level=level)
         print("[%d/%d] %s %s %s  %.1fs  entries %d" % (i + 1, len(jobs), "x".join(map(str, shape)),
                                                      ("C2C", "R2C", "C2R", "R2R")[kind], "f64" if prec else "f32",
                                                      time.time() - t0, D.tune_entries()), flush=True)

@@ -1,8 +1,8 @@
 """Every kernel of the compile-time catalog (dfft_fixed_*.hip) gives the
 oracle's result: each {2,3,5,7}-smooth length 16..512 along the contiguous
 axis, along a strided axis with outer slices and along a strided axis
-without, in both precisions, plus the fused R2C kernels of every half-length
-16..256, plus each FP64 strided length through both of its schedules
+without, in both precisions, plus the fused R2C and C2R kernels of every
+half-length 16..256, plus each FP64 strided length through both of its schedules
 (split and full LDS exchange).  ComplexF64 rel-L2 <= 1e-12; ComplexF32
 against the FP64 oracle <= 2e-6 (stated, and the worst value is printed)."""
 import math
@@ -57,6 +57,30 @@ def test_catalog_c2c_lengths(gpu, prec, part):
             B = pi.execute(F, F.similar())
             e2 = O.rel_l2(B.to_host().astype(np.complex128), a.astype(np.complex128))
             assert e2 <= TOL[prec], (shape, "inverse", e2)
+    print("worst rel-L2 (%s): %.3e" % ("f64" if prec else "f32", worst))
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+def test_catalog_c2r_half_lengths(gpu, prec):
+    """The fused C2R catalog kernels (H+1 bins -> 2H reals) of every
+    half-length 16..256: irfft of the oracle's rfft reconstructs the input."""
+    import daggerfft as D
+    from daggerfft import _lib as L
+    D.tune_clear()
+    dt = np.float64 if prec else np.float32
+    cdt = np.complex128 if prec else np.complex64
+    worst = 0.0
+    for h in HALF:
+        shape = (2 * h, 3, 2)
+        a = O.fill(h + 1, shape, np.float64)
+        F = O.rfftn(a).astype(cdt)
+        p = D.Plan(shape, L.C2R, +1, prec, D.Pencil(), (1, 2, 3))
+        q = [x for x in p.passes() if x["op"] == "C2R"][0]
+        assert q["variant"] == 3, q
+        B = p.execute(D.DeviceArray.from_host(F), D.DeviceArray(shape, dt))
+        e = O.rel_l2(B.to_host().astype(np.float64), a)
+        assert e <= TOL[prec], (shape, e)
+        worst = max(worst, e)
     print("worst rel-L2 (%s): %.3e" % ("f64" if prec else "f32", worst))
 
 
