@@ -13,6 +13,7 @@ using AMDGPU
 export fft, ifft, rfft, irfft, fft!, ifft!, Pencil, Slab, R2R
 export FFT, IFFT, RFFT, IRFFT, FFT!, IFFT!, RFFT!, IRFFT!
 export create_workspace, cleanup_workspace!, plan_fft, plan_ifft, plan_rfft, plan_irfft
+export Batch, BatchDesc, Server, submit, wait_ticket, result!, release
 
 const libdfft = joinpath(@__DIR__, "..", "libdaggerfft_hip.so")
 
@@ -146,6 +147,18 @@ function irfft(F::ROCArray{Complex{T}}, d::Integer; decomp=Pencil(), dims=nothin
 end
 
 # ---- bang API over single-device arrays (src/fft.jl:651-776) ---------------
+# The transforms tuple selects C2C or, with R2R entries, the r2r kinds per dim
+# (Slab: transforms[1] covers the 2D stage, transforms[2] is ignored,
+# src/fft.jl:735,746).
+function _bang_plan(A::ROCArray{T}, transforms, dims, decomp, sign) where {T}
+    ks = r2r_kinds(transforms, dims, decomp)
+    if ks !== nothing
+        flags = sign > 0 ? DFFT_FLAG_NORMALIZE_R2R : UInt32(0)
+        return cached((size(A), T, sign > 0 ? :ir2r : :r2r, Tuple(dims), ks, decomp),
+                      () -> Plan(size(A), DFFT_R2R, sign, prec_code(T), decomp, dims; kinds=ks, flags=flags))
+    end
+    return sign < 0 ? plan_fft(A, dims; decomp) : plan_ifft(A, dims; decomp)
+end
 struct Workspace
     src_size::Dims
     dst_size::Dims
@@ -155,17 +168,153 @@ cleanup_workspace!(::Workspace) = lock(() -> empty!(PLAN_CACHE), PLAN_LOCK)
 
 stage_inverse(transforms) = first(transforms) isa Union{IFFT,IFFT!}
 
+# Pencil fft!(C, A, B, ...): input A, result C; ifft!(C, A, B, ...): input A
+# (the spectrum), result C -- as the reference's stage chain A -> B -> C.
 function fft!(C::ROCArray, A::ROCArray, B::ROCArray, ws_AB::Workspace, ws_BC::Workspace, transforms, dims)
-    return execute!(plan_fft(A, dims; decomp=Pencil()), C, A)
+    return execute!(_bang_plan(A, transforms, Tuple(dims), Pencil(), -1), C, A)
 end
 function fft!(B::ROCArray, A::ROCArray, ws::Workspace, transforms, dims, ::Slab)
-    return execute!(plan_fft(A, dims; decomp=Slab()), B, A)
+    return execute!(_bang_plan(A, transforms, Tuple(dims), Slab(), -1), B, A)
 end
 function ifft!(C::ROCArray, A::ROCArray, B::ROCArray, ws_AB::Workspace, ws_BC::Workspace, transforms, dims)
-    return execute!(plan_ifft(A, dims; decomp=Pencil()), C, A)
+    return execute!(_bang_plan(A, transforms, Tuple(dims), Pencil(), +1), C, A)
 end
 function ifft!(A::ROCArray, B::ROCArray, ws::Workspace, transforms, dims, ::Slab)
-    return execute!(plan_ifft(B, dims; decomp=Slab()), A, B)
+    return execute!(_bang_plan(B, transforms, Tuple(dims), Slab(), +1), A, B)
 end
+
+# ---- batched transforms (dfft_batch_*; no reference counterpart) ----------
+# Many independent 3D transforms per call: each transform's own plan on four
+# streams forked from the caller's (default), or one grouped launch per
+# axis-pass level (DFFT_FLAG_BATCH_GROUPED); DFFT_FLAG_GRAPH replays the
+# launches as one hipGraph.
+const DFFT_FLAG_GRAPH = UInt32(0x10)
+const DFFT_FLAG_BATCH_GROUPED = UInt32(0x80)
+
+struct BatchDesc            # dfft_batch_desc, field for field
+    n::NTuple{3,Int64}
+    kind::Cint
+    sign::Cint
+    precision::Cint
+    flags::UInt32
+end
+BatchDesc(sz::Dims{3}, kind, sign, ::Type{T}) where {T} =
+    BatchDesc(Int64.(sz), Cint(kind), Cint(sign), Cint(prec_code(T)), UInt32(0))
+
+mutable struct Batch
+    handle::Ptr{Cvoid}
+    count::Int
+end
+
+function Batch(descs::Vector{BatchDesc}; flags=UInt32(0))
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:dfft_batch_create, libdfft), Cint, (Ptr{Ptr{Cvoid}}, Cint, Ptr{BatchDesc}, UInt32),
+                h, length(descs), descs, flags))
+    b = Batch(h[], length(descs))
+    finalizer(b) do q
+        q.handle != C_NULL && ccall((:dfft_batch_destroy, libdfft), Cint, (Ptr{Cvoid},), q.handle)
+        q.handle = C_NULL
+    end
+    return b
+end
+
+function execute!(b::Batch, outs::Vector{<:ROCArray}, ins::Vector{<:ROCArray})
+    length(ins) == length(outs) == b.count || throw(ArgumentError("batch of $(b.count) transforms"))
+    pin = Ptr{Cvoid}[pointer(x) for x in ins]
+    pout = Ptr{Cvoid}[pointer(y) for y in outs]
+    check(ccall((:dfft_batch_execute, libdfft), Cint, (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Cvoid}),
+                b.handle, pin, pout, stream_ptr()))
+    return outs
+end
+
+# ---- serving loop (dfft_server_*; no reference counterpart) ---------------
+# Requests are queued without waiting for the device, admitted under an HBM
+# budget as slots and memory free up, and overlapped on several streams;
+# plans and device memory stay resident.  The input of a request must hold
+# its data when submitted (the server's streams do not wait for the task's
+# stream: synchronize it first).
+struct ServerConfig         # dfft_server_config
+    hbm_budget::Csize_t
+    max_inflight::Cint
+    lanes::Cint
+    flags::UInt32
+    reserved::Cint
+end
+
+struct Request              # dfft_request
+    n::NTuple{3,Int64}
+    kind::Cint
+    sign::Cint
+    precision::Cint
+    decomp::Cint
+    flags::UInt32
+    req_flags::UInt32
+    seed::UInt64
+    d_in::Ptr{Cvoid}
+    d_out::Ptr{Cvoid}
+    t_arrival::Cdouble
+end
+
+struct RequestInfo          # dfft_request_info
+    state::Cint
+    status::Cint
+    lane::Cint
+    reserved::Cint
+    t_arrival::Cdouble
+    t_admit::Cdouble
+    t_done::Cdouble
+    d_result::Ptr{Cvoid}
+    result_bytes::Csize_t
+    algorithmic_bytes::Cdouble
+end
+
+mutable struct Server
+    handle::Ptr{Cvoid}
+end
+
+function Server(; hbm_budget=200 * 10^9, max_inflight=32, lanes=4)
+    cfg = Ref(ServerConfig(Csize_t(hbm_budget), Cint(max_inflight), Cint(lanes), UInt32(0), Cint(0)))
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:dfft_server_create, libdfft), Cint, (Ptr{Ptr{Cvoid}}, Ptr{ServerConfig}), h, cfg))
+    s = Server(h[])
+    finalizer(s) do q
+        q.handle != C_NULL && ccall((:dfft_server_destroy, libdfft), Cint, (Ptr{Cvoid},), q.handle)
+        q.handle = C_NULL
+    end
+    return s
+end
+
+# fft (complex input) or rfft (real input) of a 3D array, or ifft with sign=+1;
+# with `out`, the result is written there, else it stays in the server until
+# release.  Returns the ticket.
+function submit(s::Server, A::ROCArray{T,3}; out=nothing, sign=-1, decomp=Pencil()) where {T}
+    kind = T <: Real ? DFFT_R2C : DFFT_C2C
+    req = Ref(Request(Int64.(size(A)), Cint(kind), Cint(sign), Cint(prec_code(T)), Cint(decomp_code(decomp)),
+                      UInt32(0), UInt32(0), UInt64(0), pointer(A), out === nothing ? C_NULL : pointer(out), 0.0))
+    t = Ref{Int64}(0)
+    check(ccall((:dfft_server_submit, libdfft), Cint, (Ptr{Cvoid}, Ptr{Request}, Ptr{Int64}), s.handle, req, t))
+    return t[]
+end
+
+# true once the request finished (timeout < 0: wait without limit)
+function wait_ticket(s::Server, ticket::Int64; timeout=-1.0)
+    st = ccall((:dfft_server_wait, libdfft), Cint, (Ptr{Cvoid}, Int64, Cdouble), s.handle, ticket, timeout)
+    st == 7 && return false   # DFFT_ERR_NOT_READY
+    check(st)
+    info = Ref{RequestInfo}()
+    check(ccall((:dfft_server_info, libdfft), Cint, (Ptr{Cvoid}, Int64, Ptr{RequestInfo}), s.handle, ticket, info))
+    check(info[].status)
+    return true
+end
+
+# copy a finished request's result into `out` (ordered on the task's stream)
+function result!(out::ROCArray, s::Server, ticket::Int64)
+    check(ccall((:dfft_server_copy_result, libdfft), Cint, (Ptr{Cvoid}, Int64, Ptr{Cvoid}, Csize_t, Ptr{Cvoid}),
+                s.handle, ticket, pointer(out), sizeof(out), stream_ptr()))
+    return out
+end
+
+release(s::Server, ticket::Int64) =
+    check(ccall((:dfft_server_release, libdfft), Cint, (Ptr{Cvoid}, Int64), s.handle, ticket))
 
 end # module

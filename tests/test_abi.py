@@ -50,3 +50,20 @@ def test_null_handles_rejected():
     assert lib.dfft_plan_destroy(None) == L.DFFT_ERR_ARG
     assert lib.dfft_box_copy(None, None, None, None, None, None, None, 8, None) == L.DFFT_ERR_ARG
     assert lib.dfft_fill_uniform(None, 4, 1, 1, 0, None) == L.DFFT_ERR_ARG
+
+
+def test_julia_shim_binds_declared_symbols():
+    """Every entry point the Julia shim ccalls is declared in the header (the
+    shim cannot run here: no Julia in this image)."""
+    import os
+    import re
+
+    from conftest import ROOT
+    from daggerfft import _lib as L
+    jl = open(os.path.join(ROOT, "daggerfft.jl_amd", "julia", "DaggerFFTHIP.jl")).read()
+    called = set(re.findall(r"\(:(dfft_[a-z0-9_]+),\s*libdfft\)", jl))
+    assert called, "no ccall found"
+    missing = called - set(L.header_symbols())
+    assert not missing, missing
+    for name in ("dfft_plan_create", "dfft_execute", "dfft_batch_execute", "dfft_server_submit"):
+        assert name in called
