@@ -536,10 +536,19 @@ __device__ __forceinline__ void axis_tile(const AxisPass &a, const void *in, voi
         ib = line * a.in_so;
         ob = line * a.out_so;
     } else {
-        const int64_t nct = (a.nlines + TC - 1) / TC;
-        const int64_t o = tile / nct;
-        const int64_t col = (tile - o * nct) * TC + ls;
-        valid = active && col < a.nlines;
+        // The columns of all outer slices form one flat index f = tile*TC + ls
+        // (column f mod nlines of slice f / nlines), so a row whose length is
+        // not a multiple of TC (the n/2+1 columns of an R2C array) leaves no
+        // ragged tile: a tile may end one slice and begin the next.
+        const int64_t f0 = tile * TC;
+        int64_t o = f0 / a.nlines;
+        int64_t col = f0 - o * a.nlines + ls;
+        // TC < 2*nlines (the planner halves TC until it is), so col < 3*nlines
+        while (col >= a.nlines) {
+            col -= a.nlines;
+            ++o;
+        }
+        valid = active && o < a.nouter;
         ib = col + o * a.in_so;
         ob = col + o * a.out_so;
     }

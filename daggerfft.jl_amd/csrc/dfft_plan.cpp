@@ -246,7 +246,8 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
         a.lds_pitch = tc;
         pp.threads = tpl * tc;
         pp.lds = (size_t)rows * tc * lds_es;
-        pp.nblocks = ((a.nlines + tc - 1) / tc) * a.nouter;
+        // tiles run over the columns of all outer slices back to back
+        pp.nblocks = (a.nlines * a.nouter + tc - 1) / tc;
     }
     if (pp.nblocks > 0x7fffffffLL) return DFFT_ERR_UNSUPPORTED_SIZE;
     a.swz = 1;
