@@ -60,14 +60,90 @@ __device__ __forceinline__ void group_tile(const AxisPass &a, const void *in, vo
     }
 }
 
+// Compile-time C2C schedules inside grouped launches: for each length the
+// catalog schedule single-pass plans choose by default -- contiguous powers
+// of two from 64 take 8 values per thread, every other length and layout 16
+// values per thread (radices up to 16) with the split exchange (real and
+// imaginary parts in two rounds through half the LDS), which keeps each
+// tile's LDS at 37 KB or less: a launch reserves its largest item's LDS for
+// every workgroup.  The lengths are the 7-smooth multiples of 16 up to 256.
+template <class T, int L, bool CONTIG>
+struct GFix {
+    static constexpr bool P8 = CONTIG && (L & (L - 1)) == 0 && L >= 64;
+    static constexpr int EMAX = P8 ? 8 : 16;
+    static constexpr bool SPLIT = !P8;
+    using RL = typename std::conditional<P8, RadicesOf8<L>, RadicesOf16<L>>::type;
+    static constexpr int TPL = fixed_tpl<RL, EMAX>();
+};
+
+template <class T, int L, bool CONTIG>
+__device__ __forceinline__ void gfix_run(const AxisPass &a, const void *in, void *out, int64_t tile, int tid,
+                                         void *smem) {
+    using G = GFix<T, L, CONTIG>;
+    axis_tile<G::EMAX, T, OP_C2C, CONTIG, G::SPLIT, typename G::RL, G::TPL>(a, in, out, tile, tid,
+                                                                             reinterpret_cast<cx<T> *>(smem));
+}
+
+// X(index, L); entry id = 4 * index + 2 * prec + contig
+#define DFFT_GFIX_LENGTHS(X)                                                                          \
+    X(0, 16) X(1, 32) X(2, 48) X(3, 64) X(4, 80) X(5, 96) X(6, 112) X(7, 128) X(8, 144) X(9, 160)     \
+        X(10, 192) X(11, 224) X(12, 240) X(13, 256)
+
+__device__ __forceinline__ void gfix_tile(int id, const AxisPass &a, const void *in, void *out, int64_t tile,
+                                          int tid, void *smem) {
+    switch (id) {
+#define DFFT_GFIX_CASE(i, L)                                            \
+    case 4 * i + 0: gfix_run<float, L, false>(a, in, out, tile, tid, smem); break;  \
+    case 4 * i + 1: gfix_run<float, L, true>(a, in, out, tile, tid, smem); break;   \
+    case 4 * i + 2: gfix_run<double, L, false>(a, in, out, tile, tid, smem); break; \
+    case 4 * i + 3: gfix_run<double, L, true>(a, in, out, tile, tid, smem); break;
+        DFFT_GFIX_LENGTHS(DFFT_GFIX_CASE)
+#undef DFFT_GFIX_CASE
+    default: break;
+    }
+}
+
+#define DFFT_GFIX_E(T, P, L, C) \
+    {L, P, C ? 1 : 0, GFix<T, L, C>::EMAX, GFix<T, L, C>::SPLIT ? 1 : 0, GFix<T, L, C>::TPL, GFix<T, L, C>::RL::S, \
+     GFix<T, L, C>::RL::R}
+#define DFFT_GFIX_ENTRIES(i, L) \
+    DFFT_GFIX_E(float, 0, L, false), DFFT_GFIX_E(float, 0, L, true), DFFT_GFIX_E(double, 1, L, false), \
+        DFFT_GFIX_E(double, 1, L, true),
+static const GFixEntry g_gfix[] = {DFFT_GFIX_LENGTHS(DFFT_GFIX_ENTRIES)};
+#undef DFFT_GFIX_ENTRIES
+#undef DFFT_GFIX_E
+
+int gfix_lookup(int prec, int contig, int L) {
+    for (int i = 0; i < (int)(sizeof(g_gfix) / sizeof(g_gfix[0])); ++i)
+        if (g_gfix[i].L == L && g_gfix[i].prec == (prec ? 1 : 0) && g_gfix[i].contig == (contig ? 1 : 0)) return i;
+    return -1;
+}
+
+const GFixEntry &gfix_entry(int id) { return g_gfix[id]; }
+
+// Register budget: 16 waves per CU (128 VGPRs) for the 512- and 1024-thread
+// kernels; 12 waves (168 VGPRs) for the 256-thread kernel, whose 16-value
+// compile-time schedules spill at 128
 template <int THREADS>
-__global__ __launch_bounds__(THREADS) void k_group(GroupArgs g) {
+__global__ __launch_bounds__(THREADS, THREADS == GFIX_THREADS ? 3 : 1024 / THREADS) void k_group(GroupArgs g) {
     extern __shared__ __align__(16) unsigned char dfft_smem[];
     const int item = __builtin_amdgcn_readfirstlane(g.block_item[blockIdx.x]);
     const GroupItem &it = g.items[item];
     const void *in = reinterpret_cast<const unsigned char *>(g.slots[it.in_slot]) + it.in_off;
     void *out = reinterpret_cast<unsigned char *>(g.slots[it.out_slot]) + it.out_off;
     const int64_t tile = (int64_t)blockIdx.x - it.first_block;
+    if constexpr (THREADS == GFIX_THREADS) {
+        const int gfix = __builtin_amdgcn_readfirstlane(it.gfix);
+        if (gfix >= 0) {
+            // a private copy of the pass (held in registers: the compile-time
+            // schedules index it with constants only), so its fields are not
+            // reloaded after the tile's global stores, which the compiler
+            // cannot tell apart from the item table
+            const AxisPass a = it.ap;
+            gfix_tile(gfix, a, in, out, tile, (int)threadIdx.x, dfft_smem);
+            return;
+        }
+    }
     if (it.prec) group_tile<double>(it.ap, in, out, tile, (int)threadIdx.x, dfft_smem);
     else group_tile<float>(it.ap, in, out, tile, (int)threadIdx.x, dfft_smem);
 }
@@ -118,13 +194,15 @@ int group_create(GroupExec **out, const std::vector<std::vector<GroupItem>> &lev
         L.nitems = (int)lv.size();
         L.map_off = (int64_t)map.size();
         int need = 1;
+        bool fixed = false;
         for (size_t i = 0; i < lv.size(); ++i) {
             GroupItem it = lv[i];
             if (it.in_slot < 0 || it.in_slot >= GROUP_MAX_SLOTS || it.out_slot < 0 || it.out_slot >= GROUP_MAX_SLOTS ||
-                it.ntiles < 1 || it.threads < 1 || it.threads > 1024) {
+                it.ntiles < 1 || it.threads < 1 || it.threads > 1024 || it.gfix >= (int)(sizeof(g_gfix) / sizeof(g_gfix[0]))) {
                 group_destroy(g);
                 return DFFT_ERR_ARG;
             }
+            fixed = fixed || it.gfix >= 0;
             it.first_block = L.nblocks;
             for (int64_t b = 0; b < it.ntiles; ++b) map.push_back((int)i);
             L.nblocks += it.ntiles;
@@ -133,7 +211,8 @@ int group_create(GroupExec **out, const std::vector<std::vector<GroupItem>> &lev
             g->items.push_back(it);
         }
         L.threads = need <= 256 ? 256 : need <= 512 ? 512 : 1024;
-        if (L.lds > (size_t)lim || L.nblocks > 0x7fffffffLL) {
+        // the compile-time schedules are compiled into the 256-thread kernel only
+        if (L.lds > (size_t)lim || L.nblocks > 0x7fffffffLL || (fixed && L.threads != GFIX_THREADS)) {
             group_destroy(g);
             return DFFT_ERR_UNSUPPORTED_SIZE;
         }
@@ -269,6 +348,7 @@ GroupItem make_group_item(const PassPlan &pp, int in_slot, int64_t in_off, int o
     it.ntiles = pp.nblocks;
     it.threads = pp.threads;
     it.lds = pp.lds;
+    it.gfix = -1;
     return it;
 }
 
@@ -326,6 +406,67 @@ PassPlan plan_pass(const dfft_plan *p, int i);
 }
 
 constexpr int BATCH_LANES = 4;
+
+// Geometry of a pass that runs compile-time schedule e in a grouped launch:
+// as many lines (contiguous) or columns (strided) per tile as fill
+// GFIX_THREADS threads within 64 KB of LDS.  The same rules as the planner's
+// for single-pass launches.
+static bool gfix_schedule(PassPlan &pp, const GFixEntry &e) {
+    AxisPass &a = pp.ap;
+    const int es = pp.prec ? 16 : 8;
+    const int lds_es = e.split ? es / 2 : es;
+    const int rows = a.sp.L + 1;
+    const size_t lim = 65536;
+    if (e.S > MAX_STAGES) return false;
+    a.variant = 0;  // informational: the grouped kernel dispatches on the entry
+    a.sp.S = e.S;
+    for (int s = 0; s < e.S; ++s) a.sp.R[s] = e.R[s];
+    a.TPL = e.tpl;
+    int tc = std::max(1, GFIX_THREADS / e.tpl);
+    if (a.contig) {
+        a.pad_shift = pp.prec ? 3 : 4;
+        a.lds_pitch = rows + (rows >> a.pad_shift) + 1;
+        while (tc > 1 && (size_t)tc * a.lds_pitch * lds_es > lim) --tc;
+        pp.lds = (size_t)tc * a.lds_pitch * lds_es;
+        pp.nblocks = (a.nlines + tc - 1) / tc;
+    } else {
+        a.pad_shift = 30;
+        // TC < 2 * columns, which the strided tile mapping assumes
+        while (tc > 1 && tc / 2 >= a.nlines) tc /= 2;
+        while (tc > 1 && (size_t)rows * tc * lds_es > lim) tc /= 2;
+        a.lds_pitch = tc;
+        pp.lds = (size_t)rows * tc * lds_es;
+        pp.nblocks = (a.nlines * a.nouter + tc - 1) / tc;
+    }
+    a.TC = tc;
+    a.swz = 1;
+    pp.threads = e.tpl * tc;
+    return pp.threads <= GFIX_THREADS && pp.lds <= lim && pp.nblocks <= 0x7fffffffLL;
+}
+
+// Schedule of one batch pass inside a grouped launch of workgroups of up to
+// max_threads threads.  With max_threads == GFIX_THREADS a C2C pass whose
+// length the grouped kernel has a compile-time schedule for takes it (gfix =
+// its entry), with as many lines or columns per tile as fill the workgroup;
+// every other pass takes the runtime engine (gfix = -1).
+static int schedule_grouped(PassPlan &pp, uint32_t flags, int max_threads, int &gfix) {
+    gfix = -1;
+    if (max_threads == GFIX_THREADS && pp.ap.op == OP_C2C) {
+        const int id = gfix_lookup(pp.prec, pp.ap.contig, pp.ap.sp.L);
+        if (id >= 0) {
+            PassPlan q = pp;
+            if (gfix_schedule(q, gfix_entry(id))) {
+                pp = q;
+                gfix = id;
+                return DFFT_OK;
+            }
+        }
+    }
+    Tuning tn;
+    const Tuning *tune = nullptr;
+    if (!(flags & DFFT_FLAG_NO_TUNE) && tuning_lookup(pp, tn) && tn.tpl <= max_threads) tune = &tn;
+    return schedule_pass(pp, tune, max_threads, 65536, 256, true);
+}
 
 // lanes mode: builds one plan per transform and the streams/events
 static int lanes_create(dfft_batch *b, uint32_t flags) {
@@ -442,7 +583,20 @@ int dfft_batch_create(dfft_batch **batch, int count, const dfft_batch_desc *desc
     if (!b) return DFFT_ERR_OOM;
     b->count = count;
     b->descs.assign(descs, descs + count);
-    if (!(flags & DFFT_FLAG_BATCH_GROUPED)) {
+    if ((flags & DFFT_FLAG_BATCH_GROUPED) && (flags & DFFT_FLAG_BATCH_LANES)) {
+        batch_free(b);
+        return DFFT_ERR_ARG;
+    }
+    bool grouped = (flags & DFFT_FLAG_BATCH_GROUPED) != 0;
+    if (!grouped && !(flags & DFFT_FLAG_BATCH_LANES)) {
+        // default: grouped when every pass has a compile-time grouped schedule
+        grouped = true;
+        for (int t = 0; t < count && grouped; ++t) {
+            grouped = descs[t].kind == DFFT_C2C;
+            for (int i = 0; i < 3 && grouped; ++i) grouped = gfix_lookup(descs[t].precision, 1, (int)descs[t].n[i]) >= 0;
+        }
+    }
+    if (!grouped) {
         b->lanes = true;
         b->use_graph = (flags & DFFT_FLAG_GRAPH) != 0;
         int rc = lanes_create(b, flags);
@@ -500,15 +654,31 @@ int dfft_batch_create(dfft_batch **batch, int count, const dfft_batch_desc *desc
                 pp.alg_bytes = 2.0 * (d.kind == DFFT_R2C ? NC : N) * es_c;
             }
             b->alg_bytes += pp.alg_bytes;
-            Tuning tn;
-            const Tuning *tune = nullptr;
-            if (!(flags & DFFT_FLAG_NO_TUNE) && tuning_lookup(pp, tn) && tn.tpl <= 512) tune = &tn;
-            rc = schedule_pass(pp, tune, 512, 65536, 256, true);
-            if (rc != DFFT_OK) break;
-            (prec ? tt64 : tt32).add(pp.ap);
             lv[i].push_back(pp);
             lv_t[i].push_back(t);
         }
+    }
+    // schedules, level by level: the grouped kernel's compile-time schedules
+    // where it has one (256-thread tiles); if a pass of the level does not
+    // fit 256 threads, the whole level runs the runtime engine in wider
+    // workgroups
+    std::vector<std::vector<int>> lv_g(3);
+    for (int l = 0; l < 3 && rc == DFFT_OK; ++l) {
+        std::vector<PassPlan> sched = lv[l];
+        std::vector<int> gf(sched.size(), -1);
+        bool wide = false;
+        for (size_t k = 0; k < sched.size() && !wide; ++k)
+            wide = schedule_grouped(sched[k], flags, GFIX_THREADS, gf[k]) != DFFT_OK;
+        if (wide) {
+            sched = lv[l];
+            gf.assign(sched.size(), -1);
+            for (size_t k = 0; k < sched.size() && rc == DFFT_OK; ++k)
+                rc = schedule_grouped(sched[k], flags, 512, gf[k]);
+        }
+        if (rc != DFFT_OK) break;
+        for (PassPlan &pp : sched) (pp.prec ? tt64 : tt32).add(pp.ap);
+        lv[l] = sched;
+        lv_g[l] = gf;
     }
     if (rc == DFFT_OK && !tt64.tab.empty()) rc = tt64.upload(&b->d_tw64, &b->tw64_bytes);
     if (rc == DFFT_OK && !tt32.tab.empty()) rc = tt32.upload(&b->d_tw32, &b->tw32_bytes);
@@ -529,6 +699,7 @@ int dfft_batch_create(dfft_batch **batch, int count, const dfft_batch_desc *desc
             pp.ap.tw = pp.prec ? b->d_tw64 : b->d_tw32;
             const int in_slot = pp.src == BUF_IN ? 2 * t : 2 * t + 1;
             levels[l].push_back(make_group_item(pp, in_slot, 0, 2 * t + 1, 0));
+            levels[l].back().gfix = lv_g[l][k];
         }
     }
     rc = group_create(&b->g, levels);

@@ -23,7 +23,19 @@ struct GroupItem {
     int64_t first_block;      // first block of this item in its launch
     int threads;              // threads a tile needs (<= the launch's workgroup size)
     size_t lds;               // tile LDS bytes
+    int gfix;                 // >= 0: compile-time schedule gfix_entry(gfix); -1: runtime engine
 };
+
+// Compile-time C2C schedules the grouped kernel runs in its 256-thread
+// launches (see dfft_group.hip), one entry per (length, precision, layout).
+struct GFixEntry {
+    int L, prec, contig, emax, split, tpl, S;
+    const int *R;
+};
+constexpr int GFIX_THREADS = 256;
+// entry of (prec, contig, L), or -1
+int gfix_lookup(int prec, int contig, int L);
+const GFixEntry &gfix_entry(int id);
 
 struct GroupExec;
 

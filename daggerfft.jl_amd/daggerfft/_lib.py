@@ -34,6 +34,7 @@ FLAG_GRAPH = 0x10
 FLAG_STAGE_ORDER = 0x20
 FLAG_PLANE_PIPELINE = 0x40
 FLAG_BATCH_GROUPED = 0x80
+FLAG_BATCH_LANES = 0x100
 
 _lib = None
 _lock = threading.Lock()

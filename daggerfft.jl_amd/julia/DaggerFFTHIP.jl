@@ -184,12 +184,14 @@ function ifft!(A::ROCArray, B::ROCArray, ws::Workspace, transforms, dims, ::Slab
 end
 
 # ---- batched transforms (dfft_batch_*; no reference counterpart) ----------
-# Many independent 3D transforms per call: each transform's own plan on four
-# streams forked from the caller's (default), or one grouped launch per
-# axis-pass level (DFFT_FLAG_BATCH_GROUPED); DFFT_FLAG_GRAPH replays the
-# launches as one hipGraph.
+# Many independent 3D transforms per call: one grouped launch per axis-pass
+# level (DFFT_FLAG_BATCH_GROUPED), or each transform's own plan on four
+# streams forked from the caller's (DFFT_FLAG_BATCH_LANES); by default the
+# library picks (grouped when every length has a compile-time grouped
+# schedule).  DFFT_FLAG_GRAPH replays the launches as one hipGraph.
 const DFFT_FLAG_GRAPH = UInt32(0x10)
 const DFFT_FLAG_BATCH_GROUPED = UInt32(0x80)
+const DFFT_FLAG_BATCH_LANES = UInt32(0x100)
 
 struct BatchDesc            # dfft_batch_desc, field for field
     n::NTuple{3,Int64}

@@ -92,8 +92,8 @@ extern "C" {
                                           contiguous axis runs first -- same result, the passes commute */
 #define DFFT_FLAG_PLANE_PIPELINE 0x40u /* Slab: software-pipeline the 2D stage over z-plane chunks
                                           (grouped launches re-reading each chunk from the Infinity Cache) */
-#define DFFT_FLAG_BATCH_GROUPED  0x80u /* batch: one grouped launch per axis-pass level (three launches)
-                                          instead of each transform's own plan on four forked streams */
+#define DFFT_FLAG_BATCH_GROUPED  0x80u /* batch: one grouped launch per axis-pass level (three launches) */
+#define DFFT_FLAG_BATCH_LANES    0x100u /* batch: each transform's own plan on four forked streams */
 
 typedef struct dfft_plan dfft_plan;
 
@@ -210,16 +210,22 @@ int dfft_tune_entries(int *count);
  * (the reference runs one cuFFT plan per chunk, src/fftgpu.jl:118-124).
  * Each transform is rank 3 over dims (0,1,2); C2C (either sign) or R2C
  * (forward, even n[0]).  Two schedules:
- *   lanes (default)          each transform's own plan (the compile-time
+ *   DFFT_FLAG_BATCH_GROUPED  launch l runs axis pass l of every transform, so
+ *                            a whole batch takes three launches and no
+ *                            workgroup waits for another.  C2C passes of the
+ *                            lengths 16, 32, 48, 64, 80, 96, 112, 128, 144,
+ *                            160, 192, 224, 240, 256 run compile-time
+ *                            schedules in 256-thread tiles; other passes the
+ *                            runtime engine.  Results equal each transform
+ *                            alone up to rounding;
+ *   DFFT_FLAG_BATCH_LANES    each transform's own plan (the compile-time
  *                            kernels) on one of four streams forked from and
  *                            joined back into the caller's stream; results
- *                            equal each transform run alone bit for bit;
- *   DFFT_FLAG_BATCH_GROUPED  launch l runs axis pass l of every transform, so
- *                            a whole batch takes three launches (the kernel
- *                            follows the runtime schedule; results equal each
- *                            transform alone up to rounding).  Measured 2.9x
- *                            slower than lanes on the 64-transform mix: every
- *                            tile runs with the launch's largest block.
+ *                            equal each transform run alone bit for bit.
+ * Neither flag: grouped when every axis length of every transform is one of
+ * the compile-time grouped lengths and every transform is C2C (the
+ * 64-transform mix of 32..256: 2.15 ms grouped against 2.20 ms lanes), else
+ * lanes.  Both flags: DFFT_ERR_ARG.
  * With DFFT_FLAG_GRAPH at creation the launches are recorded once as a
  * hipGraph (the lanes as parallel branches) and replayed by every execute,
  * re-recorded when the buffer pointers change. */

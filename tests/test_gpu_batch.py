@@ -137,6 +137,22 @@ def test_grouped_batch_small_mix(gpu):
         assert O.rel_l2(Y.to_host(), alone.to_host()) <= tol
 
 
+def test_grouped_batch_wide_level(gpu):
+    """A 4096-point ComplexF32 contiguous pass needs 512-thread tiles: its level runs the
+    runtime engine throughout, while the other levels run the compile-time
+    schedules (the 64- and 48-point passes) beside runtime ones (4, 2, 30)."""
+    import daggerfft as D
+    from daggerfft import _lib as L
+    descs = [((4096, 4, 2), L.C2C, -1, 0), ((64, 64, 48), L.C2C, -1, 1), ((48, 30, 64), L.C2C, +1, 0)]
+    b, ins, outs = _run_grouped(descs, [61, 62, 63])
+    for t, ((shape, kind, sign, prec), X, Y) in enumerate(zip(descs, ins, outs)):
+        dt = np.complex128 if prec == 1 else np.complex64
+        a = O.fill(61 + t, shape, dt).astype(np.complex128)
+        ref = O.fftn(a, inverse=sign > 0)
+        e = O.rel_l2(Y.to_host().astype(np.complex128), ref)
+        assert e <= (1e-12 if prec == 1 else 1e-5), (t, e)
+
+
 def test_grouped_batch_config_64_mixed(gpu):
     import daggerfft as D
     from daggerfft import _lib as L
@@ -219,7 +235,7 @@ def test_grouped_batch_graph_replays(gpu):
 
 @pytest.mark.parametrize("flags", ["lanes", "lanes_graph"])
 def test_lanes_batch_equals_plans_bitwise(gpu, flags):
-    """The default (lanes) schedule: each transform runs its own plan on one of four
+    """The lanes schedule: each transform runs its own plan on one of four
     forked streams (optionally recorded as one hipGraph with parallel
     branches); results equal the plain plan of each transform bit for bit,
     over replays with new data and eager work in between."""
@@ -233,7 +249,7 @@ def test_lanes_batch_equals_plans_bitwise(gpu, flags):
     cfg = batch_config_sizes(24, 9)
     descs = [(shape, L.C2C, -1 if t % 3 else +1, prec) for t, (shape, prec) in enumerate(cfg)]
     descs += [((64, 30, 14), L.R2C, -1, 1), ((40, 21, 10), L.R2C, -1, 0)]
-    fl = L.FLAG_GRAPH if flags == "lanes_graph" else 0
+    fl = L.FLAG_BATCH_LANES | (L.FLAG_GRAPH if flags == "lanes_graph" else 0)
     b = Batch(descs, flags=fl)
     assert b.launches == 3 * len(descs)
     ins, outs = [], []
@@ -258,3 +274,24 @@ def test_lanes_batch_equals_plans_bitwise(gpu, flags):
             else:
                 alone = D.fft(X) if sign < 0 else D.ifft(X)
             assert torch.equal(Y.tensor, alone.tensor), (rep, t)
+
+
+def test_batch_default_schedule(gpu):
+    """Without a schedule flag the batch runs grouped (three launches) when
+    every transform is C2C and every axis length has a compile-time grouped
+    schedule, else lanes (three launches per transform); both flags at once
+    are refused."""
+    import ctypes
+
+    from daggerfft import _lib as L
+    from daggerfft.batch import Batch, batch_config_sizes
+    cfg = batch_config_sizes(64, 7)
+    assert Batch([(shape, L.C2C, -1, prec) for shape, prec in cfg]).launches == 3
+    assert Batch([((64, 30, 48), L.C2C, -1, 1), ((32, 32, 32), L.C2C, -1, 0)]).launches == 6
+    assert Batch([((64, 64, 64), L.R2C, -1, 1), ((32, 32, 32), L.C2C, -1, 0)]).launches == 6
+    arr = (L.dfft_batch_desc * 1)()
+    arr[0].n[0], arr[0].n[1], arr[0].n[2] = 32, 32, 32
+    arr[0].kind, arr[0].sign, arr[0].precision = L.C2C, -1, 1
+    h = ctypes.c_void_p()
+    st = L.lib().dfft_batch_create(ctypes.byref(h), 1, arr, L.FLAG_BATCH_GROUPED | L.FLAG_BATCH_LANES)
+    assert st == L.DFFT_ERR_ARG
