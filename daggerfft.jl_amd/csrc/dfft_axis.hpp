@@ -98,11 +98,21 @@ struct TileCtx {
             if (!valid) return cx<T>{0, 0};
             return cx<T>{gload(inr + ib + (int64_t)p * a.in_sm, a.nt), (T)0};
         } else if constexpr (OP == OP_C2R) {
-            // the H+1 input bins sit in LDS; Z_p = A + i w^-p B (conjugated: the
-            // inverse H-point FFT runs as a forward FFT on conjugates)
+            // Z_p = A + i w^-p B from bins p and H-p (conjugated: the inverse
+            // H-point FFT runs as a forward FFT on conjugates).  Both bins are
+            // read from global memory with cached loads: the tile reads each
+            // line forwards and backwards at the same time, so the second read
+            // of a bin hits the cache.  The DC and Nyquist imaginary parts are
+            // ignored (FFTW's c2r input has no slot for them).
+            if (!valid) return cx<T>{0, 0};
             const int H = a.sp.L;
-            cx<T> xk = lds[li(p)];
-            cx<T> xc = conj(lds[li(H - p)]);
+            cx<T> xk = gload(inc + ib + (int64_t)p * a.in_sm, false);
+            cx<T> xh = gload(inc + ib + (int64_t)(H - p) * a.in_sm, false);
+            if (p == 0) {
+                xk.y = 0;
+                xh.y = 0;
+            }
+            cx<T> xc = conj(xh);
             cx<T> A = xk + xc, B = xk - xc;
             cx<T> wB = cmul(conj(post()[p]), B);
             cx<T> z = A + mul_pos_i(wB);
@@ -227,7 +237,7 @@ __device__ __forceinline__ void run_stage(const TileCtx<T, OP, CONTIG> &c, int N
         }
     }
     // every read of the shared slots must finish before anyone overwrites them
-    if constexpr (!first || OP == OP_C2R) __syncthreads();
+    if constexpr (!first) __syncthreads();
 #pragma unroll
     for (int b = 0; b < NBM; ++b) {
         const int j = c.t + b * TPL;
@@ -485,16 +495,72 @@ __device__ __forceinline__ void fixed_stages_from(const TileCtx<T, OP, CONTIG> &
     }
 }
 
+// R2C post-twiddle with the split exchange.  The last stage leaves the
+// H-point FFT Z of the packed line in registers (Z_p at p = j + r*NB); bin k
+// needs Z_k and Z_{H-k} (indices mod H).  With Z_k = a + ib and
+// Z_{H-k} = c + id and w = w_k = e^{-i pi k/H}:
+//   X_k = 1/2 [(Z_k + conj Z_{H-k}) - i w (Z_k - conj Z_{H-k})]
+//   Re X_k = 1/2 [(a + c) + Im w (a - c) + Re w (b + d)]
+//   Im X_k = 1/2 [(b - d) - Re w (a - c) + Im w (b + d)]
+// The real parts go through LDS first and leave partial sums, then the
+// imaginary parts finish them; each thread takes the pairs (k, H-k),
+// k = t, t+TPL, ... <= H/2, which read the same four reals.
+template <int R, int EMAX, int LC, int TPLC, class T, int OP, bool CONTIG>
+__device__ __forceinline__ void r2c_post_split(const TileCtx<T, OP, CONTIG> &c, const cx<T> *v) {
+    constexpr int NBM = EMAX / R, NB = LC / R, TPL = TPLC, H = LC;
+    constexpr int NK = (H / 2 + TPL) / TPL;  // pairs k = t + i*TPL <= H/2
+    T *ldsr = reinterpret_cast<T *>(c.lds);
+    const cx<T> *pw = c.post();
+    T pr[NK][2], pim[NK][2];
+#pragma unroll
+    for (int part = 0; part < 2; ++part) {
+        // the last read of the shared slots (if any) ended with a barrier
+#pragma unroll
+        for (int b = 0; b < NBM; ++b) {
+            const int j = c.t + b * TPL;
+            if (j < NB) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) ldsr[c.li(j + r * NB)] = part ? v[b * R + r].y : v[b * R + r].x;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NK; ++i) {
+            const int k = c.t + i * TPL;
+            if (k <= H / 2) {
+                const T zk = ldsr[c.li(k)], zh = ldsr[c.li(k == 0 ? 0 : H - k)];
+                const cx<T> wk = pw[k], wh = pw[H - k];
+                if (part == 0) {  // zk = a, zh = c
+                    pr[i][0] = (zk + zh) + wk.y * (zk - zh);
+                    pim[i][0] = -wk.x * (zk - zh);
+                    pr[i][1] = (zh + zk) + wh.y * (zh - zk);
+                    pim[i][1] = -wh.x * (zh - zk);
+                } else {  // zk = b, zh = d
+                    const T s = (T)0.5 * (T)c.a.scale;
+                    const cx<T> xk{s * (pr[i][0] + wk.x * (zk + zh)), s * (pim[i][0] + (zk - zh) + wk.y * (zk + zh))};
+                    const cx<T> xh{s * (pr[i][1] + wh.x * (zh + zk)), s * (pim[i][1] + (zh - zk) + wh.y * (zh + zk))};
+                    if (c.valid) {
+                        cx<T> *outc = reinterpret_cast<cx<T> *>(c.out);
+                        gstore(outc + c.ob + (int64_t)k * c.a.out_sm, xk, c.a.nt);
+                        if (H - k != k) gstore(outc + c.ob + (int64_t)(H - k) * c.a.out_sm, xh, c.a.nt);
+                    }
+                }
+            }
+        }
+        if (part == 0) __syncthreads();
+    }
+}
+
 // All stages of a fixed schedule: global -> registers, LDS exchanges, registers -> global.
 template <class RL, int EMAX, int TPLC, bool SPLIT, class T, int OP, bool CONTIG>
 __device__ __forceinline__ void run_fixed_stages(const TileCtx<T, OP, CONTIG> &c) {
     constexpr int L = RL::L;
     cx<T> v[EMAX];
     st_load_global<RL::R[0], EMAX, L, TPLC>(c, v);
-    if constexpr (OP == OP_C2R) __syncthreads();
     st_compute<RL::R[0], EMAX, L, TPLC, 1>(c, v, 0);
     fixed_stages_from<RL, EMAX, TPLC, SPLIT, 0>(c, v);
-    st_store_global<RL::R[RL::S - 1], EMAX, L, TPLC>(c, v);
+    if constexpr (OP == OP_R2C && SPLIT) r2c_post_split<RL::R[RL::S - 1], EMAX, L, TPLC>(c, v);
+    else st_store_global<RL::R[RL::S - 1], EMAX, L, TPLC>(c, v);
 }
 
 // Runs tile `tile` of pass `a` with the calling workgroup.  Threads with
@@ -555,17 +621,6 @@ __device__ __forceinline__ void axis_tile(const AxisPass &a, const void *in, voi
     TileCtx<T, OP, CONTIG> c{a, in, out, lds, ls, t, valid, ib, ob};
     const StagePlan &sp = a.sp;
 
-    if constexpr (OP == OP_C2R) {
-        // stage the H+1 input bins; the DC and Nyquist imaginary parts are ignored
-        const int H = sp.L;
-        const cx<T> *inc = reinterpret_cast<const cx<T> *>(in);
-        for (int k = t; k <= H; k += TPL) {
-            cx<T> x = valid ? gload(inc + ib + (int64_t)k * a.in_sm, a.nt) : cx<T>{0, 0};
-            if (k == 0 || k == H) x.y = 0;
-            lds[c.li(k)] = x;
-        }
-        __syncthreads();
-    }
 
     if (sp.S == 0) {  // length-1 transform: only the fused load/store maps apply
         if (t == 0) c.store_last(0, c.load0(0));
@@ -584,7 +639,7 @@ __device__ __forceinline__ void axis_tile(const AxisPass &a, const void *in, voi
         dispatch_stage<SK_LAST, EMAX>(c, sp.R[sp.S - 1], Ns, sp.tw[sp.S - 1]);
     }
 
-    if constexpr (OP == OP_R2C) {
+    if constexpr (OP == OP_R2C && !(FIXED && SPLIT)) {
         // X_k = 1/2 [(Z_k + conj Z_{H-k}) - i w^k (Z_k - conj Z_{H-k})], k = 0..H
         __syncthreads();
         const int H = sp.L;

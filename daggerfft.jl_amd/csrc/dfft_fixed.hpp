@@ -8,8 +8,7 @@
 // Variants (AxisPass::variant):
 //   1  8 values per thread, whole complex values exchanged through LDS
 //   2  16 values per thread, real and imaginary parts exchanged in two
-//      rounds through half the LDS (C2C and DCT-II/III; the fused R2C/C2R
-//      steps keep whole complex values in LDS)
+//      rounds through half the LDS (C2C, R2C, C2R and DCT-II/III)
 //   3  16 values per thread, whole complex values exchanged through LDS
 //      (the mixed-radix lengths: radix-3/5/7 stages leave at most 1-2 of
 //      the 16 registers idle, against 3 of 8)
@@ -55,15 +54,25 @@ struct FixedTable {
         DFFT_FX(T, P, OP_C2C, 0, 2, 16, true, RadicesOf16<L>)
 #define DFFT_C2C_F64(L) DFFT_C2C_16(double, 1, L)
 #define DFFT_C2C_F32(L) DFFT_C2C_16(float, 0, L)
-// R2C of a real line of 2H points through an H-point FFT (contiguous axis)
-#define DFFT_R2C(H)                                                    \
+// R2C of a real line of 2H points through an H-point FFT (contiguous axis):
+// full (3) and split (2) LDS exchange.  The split schedule runs the
+// post-twiddle in two rounds (real parts, then imaginary parts) and keeps two
+// partial sums per output bin of the thread between them, so it is built
+// only where a line has enough threads (H >= 100: 7 or more) to hold them in
+// registers; below that the FP64 kernels spill.
+#define DFFT_R2C_FULL(H)                                               \
     DFFT_FX(double, 1, OP_R2C, 1, 3, 16, false, RadicesOf16<H>),    \
         DFFT_FX(float, 0, OP_R2C, 1, 3, 16, false, RadicesOf16<H>)
-// C2R of H+1 bins to a real line of 2H points (contiguous axis); the bins
-// are staged in LDS as whole complex values, so the full exchange (3)
+#define DFFT_R2C(H)                                                    \
+    DFFT_R2C_FULL(H), DFFT_FX(double, 1, OP_R2C, 1, 2, 16, true, RadicesOf16<H>), \
+        DFFT_FX(float, 0, OP_R2C, 1, 2, 16, true, RadicesOf16<H>)
+// C2R of H+1 bins to a real line of 2H points (contiguous axis): the
+// pre-twiddle reads bins p and H-p from global memory, so both exchanges apply
 #define DFFT_C2R(H)                                                    \
     DFFT_FX(double, 1, OP_C2R, 1, 3, 16, false, RadicesOf16<H>),    \
-        DFFT_FX(float, 0, OP_C2R, 1, 3, 16, false, RadicesOf16<H>)
+        DFFT_FX(float, 0, OP_C2R, 1, 3, 16, false, RadicesOf16<H>), \
+        DFFT_FX(double, 1, OP_C2R, 1, 2, 16, true, RadicesOf16<H>), \
+        DFFT_FX(float, 0, OP_C2R, 1, 2, 16, true, RadicesOf16<H>)
 
 // DCT-II / DCT-III (REDFT10 / REDFT01, Makhoul's n-point algorithm) of
 // length L, split LDS exchange, both layouts: their fused load and store maps

@@ -2,8 +2,8 @@
 oracle's result: each {2,3,5,7}-smooth length 16..512 along the contiguous
 axis, along a strided axis with outer slices and along a strided axis
 without, in both precisions, plus the fused R2C and C2R kernels of every
-half-length 16..256, plus each FP64 strided length through both of its schedules
-(split and full LDS exchange).  ComplexF64 rel-L2 <= 1e-12; ComplexF32
+half-length 16..256 through each of their schedules, plus each FP64 strided
+length through both of its schedules (split and full LDS exchange).  ComplexF64 rel-L2 <= 1e-12; ComplexF32
 against the FP64 oracle <= 2e-6 (stated, and the worst value is printed)."""
 import math
 import os
@@ -60,47 +60,73 @@ def test_catalog_c2c_lengths(gpu, prec, part):
     print("worst rel-L2 (%s): %.3e" % ("f64" if prec else "f32", worst))
 
 
+def _force(tmp_path, line):
+    import daggerfft as D
+    path = os.path.join(str(tmp_path), "tune.txt")
+    with open(path, "w") as f:
+        f.write(line + "\n")
+    D.tune_clear()
+    D.tune_load(path)
+
+
 @pytest.mark.parametrize("prec", [1, 0])
-def test_catalog_c2r_half_lengths(gpu, prec):
+def test_catalog_c2r_half_lengths(gpu, tmp_path, prec):
     """The fused C2R catalog kernels (H+1 bins -> 2H reals) of every
-    half-length 16..256: irfft of the oracle's rfft reconstructs the input."""
+    half-length 16..256, with the split (2) and the full (3) LDS exchange
+    forced through the schedule cache: irfft of the oracle's rfft
+    reconstructs the input."""
     import daggerfft as D
     from daggerfft import _lib as L
-    D.tune_clear()
     dt = np.float64 if prec else np.float32
     cdt = np.complex128 if prec else np.complex64
     worst = 0.0
-    for h in HALF:
-        shape = (2 * h, 3, 2)
-        a = O.fill(h + 1, shape, np.float64)
-        F = O.rfftn(a).astype(cdt)
-        p = D.Plan(shape, L.C2R, +1, prec, D.Pencil(), (1, 2, 3))
-        q = [x for x in p.passes() if x["op"] == "C2R"][0]
-        assert q["variant"] == 3, q
-        B = p.execute(D.DeviceArray.from_host(F), D.DeviceArray(shape, dt))
-        e = O.rel_l2(B.to_host().astype(np.float64), a)
-        assert e <= TOL[prec], (shape, e)
-        worst = max(worst, e)
+    try:
+        for h in HALF:
+            shape = (2 * h, 3, 2)
+            a = O.fill(h + 1, shape, np.float64)
+            F = O.rfftn(a).astype(cdt)
+            sc = int(math.floor(math.log2(6 * h)))
+            for variant in (2, 3):
+                print("C2R", "f64" if prec else "f32", shape, variant, flush=True)
+                _force(tmp_path, "%d/3/%d/1/%d - 0 4 0 %d" % (prec, h, sc, variant))
+                p = D.Plan(shape, L.C2R, +1, prec, D.Pencil(), (1, 2, 3))
+                q = [x for x in p.passes() if x["op"] == "C2R"][0]
+                assert q["variant"] == variant, q
+                B = p.execute(D.DeviceArray.from_host(F), D.DeviceArray(shape, dt))
+                e = O.rel_l2(B.to_host().astype(np.float64), a)
+                assert e <= TOL[prec], (shape, variant, e)
+                worst = max(worst, e)
+    finally:
+        D.tune_clear()
     print("worst rel-L2 (%s): %.3e" % ("f64" if prec else "f32", worst))
 
 
 @pytest.mark.parametrize("prec", [1, 0])
-def test_catalog_r2c_half_lengths(gpu, prec):
+def test_catalog_r2c_half_lengths(gpu, tmp_path, prec):
+    """The fused R2C catalog kernels of every half-length 16..256: the full
+    (3) LDS exchange for all, and the split one (2, post-twiddle in two
+    rounds) for H >= 100, each forced through the schedule cache."""
     import daggerfft as D
     from daggerfft import _lib as L
-    D.tune_clear()
     dt = np.float64 if prec else np.float32
     worst = 0.0
-    for h in HALF:
-        shape = (2 * h, 3, 2)
-        print("R2C", "f64" if prec else "f32", shape, flush=True)
-        a = O.fill(h, shape, dt)
-        p = D.Plan(shape, L.R2C, -1, prec, D.Pencil(), (1, 2, 3))
-        assert p.passes()[0]["variant"] == 3 and p.passes()[0]["op"] == "R2C"
-        F = D.rfft(D.DeviceArray.from_host(a))
-        e = O.rel_l2(F.to_host().astype(np.complex128), O.rfftn(a.astype(np.float64)))
-        assert e <= TOL[prec], (shape, e)
-        worst = max(worst, e)
+    try:
+        for h in HALF:
+            shape = (2 * h, 3, 2)
+            a = O.fill(h, shape, dt)
+            ref = O.rfftn(a.astype(np.float64))
+            sc = int(math.floor(math.log2(6 * h)))
+            for variant in ((2, 3) if h >= 100 else (3,)):
+                print("R2C", "f64" if prec else "f32", shape, variant, flush=True)
+                _force(tmp_path, "%d/1/%d/1/%d - 0 4 0 %d" % (prec, h, sc, variant))
+                p = D.Plan(shape, L.R2C, -1, prec, D.Pencil(), (1, 2, 3))
+                assert p.passes()[0]["variant"] == variant and p.passes()[0]["op"] == "R2C"
+                F = p.execute(D.DeviceArray.from_host(a), D.DeviceArray((h + 1, 3, 2), np.complex128 if prec else np.complex64))
+                e = O.rel_l2(F.to_host().astype(np.complex128), ref)
+                assert e <= TOL[prec], (shape, variant, e)
+                worst = max(worst, e)
+    finally:
+        D.tune_clear()
     print("worst rel-L2 (%s): %.3e" % ("f64" if prec else "f32", worst))
 
 
