@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import sys
 import threading
 import time
 from dataclasses import dataclass
@@ -277,6 +278,12 @@ def replay(server, trace, open_loop=True, keep=None, time_scale=1.0):
             with lock:
                 done_count[0] += 1
 
+    # The submitter and the reaper share the interpreter lock.  With the
+    # default 5 ms switch interval a saturating submit loop keeps the reaper
+    # from releasing finished results for milliseconds at a time, so they pile
+    # up in the server's memory; a 0.1 ms interval interleaves the two.
+    switch = sys.getswitchinterval()
+    sys.setswitchinterval(1e-4)
     th = threading.Thread(target=reaper, daemon=True)
     th.start()
     # open loop: the first arrival is 50 ms ahead, so the submitter is ready for it
@@ -300,6 +307,7 @@ def replay(server, trace, open_loop=True, keep=None, time_scale=1.0):
         with lock:
             expected[0] = submitted
         th.join()
+        sys.setswitchinterval(switch)
     t_end = server.clock()
     if errors:
         raise DFFTError(errors[0][1], "serving request %d" % errors[0][0])
