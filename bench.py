@@ -206,6 +206,30 @@ def configs():
                                   "GFLOPs": round(5 * N * math.log2(N) / ms / 1e6, 1),
                                   "passes": _pass_table(pf, A, X), "roundtrip_rel_l2": rel(Y.tensor, A.tensor)}
     del A, X, Y
+    # 512^3 ComplexF32 C2C fft + ifft (Pencil): the precision of the reference's
+    # own benchmarks (benchmarks/fftmpi.jl:15, N = 512); FP64 twiddles, error
+    # stated against the FP64 transform of the same (widened) inputs
+    sh = (512, 512, 512)
+    A = D.rand(sh, np.complex64, seed=3)
+    X, Y = A.similar(), A.similar()
+    pf = D.Plan(sh, L.C2C, -1, L.F32, D.Pencil(), (1, 2, 3))
+    pi = D.Plan(sh, L.C2C, +1, L.F32, D.Pencil(), (1, 2, 3))
+    ms = _time_events(lambda: (pf.execute(A, X), pi.execute(X, Y)), 10) / 2
+    N = float(np.prod(sh))
+    rt32 = rel(Y.tensor.to(torch.complex128), A.tensor.to(torch.complex128))
+    A64 = D.DeviceArray(sh, np.complex128, tensor=A.tensor.to(torch.complex128))
+    del Y
+    X64 = D.fft(A64)
+    fwd32 = rel(X.tensor.to(torch.complex128), X64.tensor)
+    del A64, X64
+    out["c2c_512cubed_f32"] = {"ms_per_transform": round(ms, 4),
+                               "GBps": round(pf.algorithmic_bytes() / ms / 1e6, 1),
+                               "GFLOPs": round(5 * N * math.log2(N) / ms / 1e6, 1),
+                               "hbm_frac": round(pf.algorithmic_bytes() / ms / 1e6 / HBM_PEAK_GBPS, 4),
+                               "fwd_rel_l2_vs_f64": fwd32, "roundtrip_rel_l2": rt32,
+                               "passes": _pass_table(pf, A, X)}
+    del A, X
+    torch.cuda.empty_cache()
     # 64 mixed C2C transforms (seed 7): grouped launches, eager and hipGraph replay
     cfg = batch_config_sizes(64, 7)
     descs = [(shape, L.C2C, -1, prec) for shape, prec in cfg]
@@ -302,6 +326,21 @@ def serving(count):
         worst[k] = max(worst[k], e)
     out["served_vs_sequential"] = {"sampled": len(keep), "bit_identical": same, "max_rel_l2": worst,
                                    "tolerance": {"f64": 1e-12, "f32": 1e-6}}
+    # the ComplexF32 requests' error against the FP64 transform of the same
+    # (widened) inputs
+    worst32 = 0.0
+    for r in keep:
+        q = trace[r]
+        if q.precision != L.F32:
+            continue
+        x = D.rand(q.shape, q.in_dtype, seed=q.seed)
+        x64 = D.DeviceArray(q.shape, np.float64 if q.kind == L.R2C else np.complex128,
+                            tensor=x.tensor.to(torch.float64 if q.kind == L.R2C else torch.complex128))
+        y64 = D.rfft(x64) if q.kind == L.R2C else D.fft(x64)
+        a = served[r].tensor.to(torch.complex128)
+        worst32 = max(worst32, float(torch.linalg.vector_norm(a - y64.tensor) / torch.linalg.vector_norm(y64.tensor)))
+        del x, x64, y64
+    out["served_f32_rel_l2_vs_f64_max"] = worst32
     D.clear_plan_cache()
     torch.cuda.empty_cache()
     return out
