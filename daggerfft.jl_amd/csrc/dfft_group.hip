@@ -121,18 +121,19 @@ int gfix_lookup(int prec, int contig, int L) {
 
 const GFixEntry &gfix_entry(int id) { return g_gfix[id]; }
 
-// Register budget: 16 waves per CU (128 VGPRs) for the 512- and 1024-thread
-// kernels; 12 waves (168 VGPRs) for the 256-thread kernel, whose 16-value
-// compile-time schedules spill at 128
-template <int THREADS>
-__global__ __launch_bounds__(THREADS, THREADS == GFIX_THREADS ? 3 : 1024 / THREADS) void k_group(GroupArgs g) {
+// FIX: the launch holds compile-time schedules (GFIX_THREADS threads only);
+// its register budget is 12 waves per CU (168 VGPRs): the 16-value schedules
+// spill at 128.  Launches of runtime-engine items only take the kernel
+// without them, whose register count follows from the runtime engine alone.
+template <int THREADS, bool FIX>
+__global__ __launch_bounds__(THREADS, FIX ? 3 : 1) void k_group(GroupArgs g) {
     extern __shared__ __align__(16) unsigned char dfft_smem[];
     const int item = __builtin_amdgcn_readfirstlane(g.block_item[blockIdx.x]);
     const GroupItem &it = g.items[item];
     const void *in = reinterpret_cast<const unsigned char *>(g.slots[it.in_slot]) + it.in_off;
     void *out = reinterpret_cast<unsigned char *>(g.slots[it.out_slot]) + it.out_off;
     const int64_t tile = (int64_t)blockIdx.x - it.first_block;
-    if constexpr (THREADS == GFIX_THREADS) {
+    if constexpr (FIX) {
         const int gfix = __builtin_amdgcn_readfirstlane(it.gfix);
         if (gfix >= 0) {
             // a private copy of the pass (held in registers: the compile-time
@@ -155,6 +156,7 @@ struct GroupLaunch {
     int first_item = 0, nitems = 0;
     int64_t nblocks = 0;
     int threads = 256;
+    bool fixed = false;  // holds compile-time schedules (k_group<GFIX_THREADS, true>)
     size_t lds = 0;
     int64_t map_off = 0;  // offset of this launch's block -> item map
 };
@@ -173,11 +175,12 @@ void group_destroy(GroupExec *g) {
     delete g;
 }
 
-static const void *group_kernel(int threads) {
+static const void *group_kernel(int threads, bool fixed) {
+    if (fixed) return (const void *)k_group<GFIX_THREADS, true>;
     switch (threads) {
-    case 1024: return (const void *)k_group<1024>;
-    case 512: return (const void *)k_group<512>;
-    default: return (const void *)k_group<256>;
+    case 1024: return (const void *)k_group<1024, false>;
+    case 512: return (const void *)k_group<512, false>;
+    default: return (const void *)k_group<256, false>;
     }
 }
 
@@ -212,6 +215,7 @@ int group_create(GroupExec **out, const std::vector<std::vector<GroupItem>> &lev
         }
         L.threads = need <= 256 ? 256 : need <= 512 ? 512 : 1024;
         // the compile-time schedules are compiled into the 256-thread kernel only
+        L.fixed = fixed;
         if (L.lds > (size_t)lim || L.nblocks > 0x7fffffffLL || (fixed && L.threads != GFIX_THREADS)) {
             group_destroy(g);
             return DFFT_ERR_UNSUPPORTED_SIZE;
@@ -234,7 +238,7 @@ int group_create(GroupExec **out, const std::vector<std::vector<GroupItem>> &lev
         return DFFT_ERR_HIP;
     }
     for (const auto &L : g->launches)
-        if (L.lds > 65536 && hipFuncSetAttribute(group_kernel(L.threads), hipFuncAttributeMaxDynamicSharedMemorySize,
+        if (L.lds > 65536 && hipFuncSetAttribute(group_kernel(L.threads, L.fixed), hipFuncAttributeMaxDynamicSharedMemorySize,
                                                  lim) != hipSuccess) {
             group_destroy(g);
             return DFFT_ERR_HIP;
@@ -250,12 +254,15 @@ int group_launch(const GroupExec *g, void *const *slots, int nslots, hipStream_t
     for (const auto &L : g->launches) {
         a.items = g->d_items + L.first_item;
         a.block_item = g->d_map + L.map_off;
-        if (L.threads == 1024)
-            hipLaunchKernelGGL(k_group<1024>, dim3((unsigned)L.nblocks), dim3(1024), L.lds, st, a);
+        if (L.fixed)
+            hipLaunchKernelGGL((k_group<GFIX_THREADS, true>), dim3((unsigned)L.nblocks), dim3(GFIX_THREADS), L.lds, st,
+                               a);
+        else if (L.threads == 1024)
+            hipLaunchKernelGGL((k_group<1024, false>), dim3((unsigned)L.nblocks), dim3(1024), L.lds, st, a);
         else if (L.threads == 512)
-            hipLaunchKernelGGL(k_group<512>, dim3((unsigned)L.nblocks), dim3(512), L.lds, st, a);
+            hipLaunchKernelGGL((k_group<512, false>), dim3((unsigned)L.nblocks), dim3(512), L.lds, st, a);
         else
-            hipLaunchKernelGGL(k_group<256>, dim3((unsigned)L.nblocks), dim3(256), L.lds, st, a);
+            hipLaunchKernelGGL((k_group<256, false>), dim3((unsigned)L.nblocks), dim3(256), L.lds, st, a);
         if (hipError_t e = hipGetLastError()) return hip_fail(e, "grouped launch");
     }
     return DFFT_OK;
@@ -302,7 +309,7 @@ int group_launch_graph(const GroupExec *g, GroupGraph *gg, void *const *slots, i
             a.block_item = g->d_map + L.map_off;
             void *kargs[] = {&a};
             hipKernelNodeParams kp{};
-            kp.func = const_cast<void *>(group_kernel(L.threads));
+            kp.func = const_cast<void *>(group_kernel(L.threads, L.fixed));
             kp.gridDim = dim3((unsigned)L.nblocks);
             kp.blockDim = dim3(L.threads);
             kp.sharedMemBytes = (unsigned)L.lds;
