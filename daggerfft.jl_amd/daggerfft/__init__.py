@@ -32,6 +32,7 @@ __all__ = [
     "R2R", "DHT", "REDFT00", "REDFT01", "REDFT10", "REDFT11", "RODFT00", "RODFT01", "RODFT10", "RODFT11",
     "fft", "ifft", "rfft", "irfft", "fft_", "ifft_", "plan_fft", "plan_ifft", "plan_rfft", "plan_irfft",
     "plan_r2r", "Plan", "create_workspace", "cleanup_workspace", "DFFTError", "rand", "version",
+    "DArray", "compute_overlap",
 ]
 
 # ---------------------------------------------------------------------------
@@ -498,6 +499,8 @@ class Workspace:
 
 
 def create_workspace(src, dst, use_hierarchical=True, ranks_per_node=0):
+    if isinstance(src, chunked.DArray) or isinstance(dst, chunked.DArray):
+        return chunked.create_workspace(src, dst, use_hierarchical, ranks_per_node)
     return Workspace(src, dst)
 
 
@@ -525,7 +528,11 @@ def fft_(*args, stream=None):
     """fft!(C, A, B, ws_AB, ws_BC, transforms, dims) [Pencil] or
     fft!(B, A, ws, transforms, dims, Slab()).  Returns the output array; the
     stage array A is the input (the reference also leaves stage results in
-    A/B, which this single-device version does not)."""
+    A/B, which this single-chunk version does not).  Chunked arrays
+    (chunked.DArray) take the reference's per-chunk stages and box-copy
+    redistributions (daggerfft.chunked.fft_)."""
+    if args and isinstance(args[0], chunked.DArray):
+        return chunked.fft_(*args, stream=stream)
     if len(args) == 7:
         C, A, B, ws_ab, ws_bc, transforms, dims = args
         p = _stage_plan(A, transforms, tuple(dims), Pencil(), False)
@@ -539,7 +546,10 @@ def fft_(*args, stream=None):
 
 def ifft_(*args, stream=None):
     """ifft!(C, A, B, ws_AB, ws_BC, transforms, dims) [Pencil, input A, output C]
-    or ifft!(A, B, ws, transforms, dims, Slab()) [input B, output A]."""
+    or ifft!(A, B, ws, transforms, dims, Slab()) [input B, output A]; chunked
+    arrays as in fft_."""
+    if args and isinstance(args[0], chunked.DArray):
+        return chunked.ifft_(*args, stream=stream)
     if len(args) == 7:
         C, A, B, ws_ab, ws_bc, transforms, dims = args
         p = _stage_plan(A, transforms, tuple(dims), Pencil(), True)
@@ -553,3 +563,8 @@ def ifft_(*args, stream=None):
 
 gpu_fft_ = fft_
 gpu_ifft_ = ifft_
+
+
+# chunked arrays (the reference's Blocks-distributed DArray on one device)
+from . import chunked  # noqa: E402
+from .chunked import DArray, compute_overlap  # noqa: E402
