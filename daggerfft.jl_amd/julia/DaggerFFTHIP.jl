@@ -13,6 +13,7 @@ using AMDGPU
 export fft, ifft, rfft, irfft, fft!, ifft!, Pencil, Slab, R2R
 export FFT, IFFT, RFFT, IRFFT, FFT!, IFFT!, RFFT!, IRFFT!
 export create_workspace, cleanup_workspace!, plan_fft, plan_ifft, plan_rfft, plan_irfft
+export BlockedArray, distribute, compute_overlap, redistribute!
 export Batch, BatchDesc, Server, submit, wait_ticket, result!, release
 
 const libdfft = joinpath(@__DIR__, "..", "libdaggerfft_hip.so")
@@ -181,6 +182,131 @@ function ifft!(C::ROCArray, A::ROCArray, B::ROCArray, ws_AB::Workspace, ws_BC::W
 end
 function ifft!(A::ROCArray, B::ROCArray, ws::Workspace, transforms, dims, ::Slab)
     return execute!(_bang_plan(B, transforms, Tuple(dims), Slab(), +1), A, B)
+end
+
+# ---- bang API over Blocks-distributed arrays on one device ----------------
+# A BlockedArray is a DArray with Blocks(b1,b2,b3) whose chunks all live on
+# this GPU.  create_workspace keeps the local patterns of the reference's
+# create_workspace (src/fft.jl:233-343, the branch at :301-305),
+# compute_overlap is src/fft.jl:176-178, every pattern is moved by
+# dfft_box_copy (local_copy!, src/fft.jl:216-231), and the stage drivers
+# follow src/fft.jl:651-776 with per-chunk in-place transforms (apply_fft!).
+struct BlockedArray{T}
+    size::NTuple{3,Int}
+    subdomains::Vector{NTuple{3,UnitRange{Int}}}
+    chunks::Vector{ROCArray{T,3}}
+end
+
+function BlockedArray{T}(sz::NTuple{3,Int}, blocks::NTuple{3,Int}) where {T}
+    b = ntuple(i -> clamp(blocks[i], 1, sz[i]), 3)
+    doms = NTuple{3,UnitRange{Int}}[]
+    for k0 in 1:b[3]:sz[3], j0 in 1:b[2]:sz[2], i0 in 1:b[1]:sz[1]   # dim 1 fastest
+        push!(doms, (i0:min(i0 + b[1] - 1, sz[1]), j0:min(j0 + b[2] - 1, sz[2]), k0:min(k0 + b[3] - 1, sz[3])))
+    end
+    return BlockedArray{T}(sz, doms, [ROCArray{T}(undef, length.(d)) for d in doms])
+end
+
+# dst[dlo .+ (0:cnt-1)...] = src[slo .+ (0:cnt-1)...]; 1-based lo as in Julia
+function box_copy!(dst::ROCArray{T,3}, dlo, src::ROCArray{T,3}, slo, cnt) where {T}
+    any(==(0), cnt) && return dst
+    check(ccall((:dfft_box_copy, libdfft), Cint,
+                (Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Cint, Ptr{Cvoid}),
+                pointer(dst), Int64[size(dst)...], Int64[(dlo .- 1)...], pointer(src), Int64[size(src)...],
+                Int64[(slo .- 1)...], Int64[cnt...], sizeof(T), stream_ptr()))
+    return dst
+end
+
+function distribute(A::ROCArray{T,3}, blocks::NTuple{3,Int}) where {T}
+    D = BlockedArray{T}(size(A), blocks)
+    for (d, ch) in zip(D.subdomains, D.chunks)
+        box_copy!(ch, (1, 1, 1), A, first.(d), length.(d))
+    end
+    return D
+end
+
+function Base.collect(D::BlockedArray{T}) where {T}
+    out = ROCArray{T}(undef, D.size)
+    for (d, ch) in zip(D.subdomains, D.chunks)
+        box_copy!(out, first.(d), ch, (1, 1, 1), length.(d))
+    end
+    return out
+end
+
+compute_overlap(src_domain, dst_domain) = ntuple(i -> intersect(src_domain[i], dst_domain[i]), 3)
+
+struct CoalescedPattern          # src/fft.jl:66-75 (one-device fields)
+    src_idx::Int
+    dst_idx::Int
+    overlap::NTuple{3,UnitRange{Int}}
+    src_indices::NTuple{3,UnitRange{Int}}
+    dst_indices::NTuple{3,UnitRange{Int}}
+    buffer_offset::Int
+    buffer_size::Int
+end
+
+struct BlockedWorkspace
+    local_patterns::Vector{CoalescedPattern}
+end
+cleanup_workspace!(::BlockedWorkspace) = lock(() -> empty!(PLAN_CACHE), PLAN_LOCK)
+
+function create_workspace(src::BlockedArray{T}, dst::BlockedArray{T}; kwargs...) where {T}
+    src.size == dst.size || throw(ArgumentError("create_workspace: arrays must share their size"))
+    pats = CoalescedPattern[]
+    for (dst_idx, dd) in enumerate(dst.subdomains), (src_idx, sd) in enumerate(src.subdomains)
+        ov = compute_overlap(sd, dd)
+        any(isempty, ov) && continue
+        si = ntuple(i -> ov[i] .- first(sd[i]) .+ 1, 3)
+        di = ntuple(i -> ov[i] .- first(dd[i]) .+ 1, 3)
+        push!(pats, CoalescedPattern(src_idx, dst_idx, ov, si, di, 0, prod(length.(ov))))
+    end
+    return BlockedWorkspace(pats)
+end
+
+function redistribute!(dst::BlockedArray, src::BlockedArray, ws::BlockedWorkspace)
+    for p in ws.local_patterns
+        box_copy!(dst.chunks[p.dst_idx], first.(p.dst_indices), src.chunks[p.src_idx], first.(p.src_indices),
+                  length.(p.overlap))
+    end
+    return dst
+end
+
+function apply_fft!(chunk::ROCArray{T}, transform, dims) where {T}
+    d = dims isa Tuple ? dims : (dims,)
+    if transform isa R2R
+        ks = length(transform.kinds) == length(d) ? transform.kinds : ntuple(_ -> transform.kinds[1], length(d))
+        p = cached((size(chunk), T, :r2r, d, ks, Pencil()),
+                   () -> Plan(size(chunk), DFFT_R2R, -1, prec_code(T), Pencil(), d; kinds=ks))
+    elseif transform isa Union{FFT,FFT!}
+        p = plan_fft(chunk, d)
+    elseif transform isa Union{IFFT,IFFT!}
+        p = plan_ifft(chunk, d)
+    else
+        throw(ArgumentError("Unknown transform type"))
+    end
+    return execute!(p, chunk, chunk)
+end
+
+stage!(D::BlockedArray, transform, dims) = (foreach(ch -> apply_fft!(ch, transform, dims), D.chunks); D)
+
+function fft!(C::BlockedArray, A::BlockedArray, B::BlockedArray, ws_AB::BlockedWorkspace, ws_BC::BlockedWorkspace,
+              transforms, dims)
+    stage!(A, transforms[1], dims[1]); redistribute!(B, A, ws_AB)
+    stage!(B, transforms[2], dims[2]); redistribute!(C, B, ws_BC)
+    return stage!(C, transforms[3], dims[3])
+end
+function ifft!(C::BlockedArray, A::BlockedArray, B::BlockedArray, ws_AB::BlockedWorkspace, ws_BC::BlockedWorkspace,
+               transforms, dims)
+    stage!(A, transforms[3], dims[3]); redistribute!(B, A, ws_AB)
+    stage!(B, transforms[2], dims[2]); redistribute!(C, B, ws_BC)
+    return stage!(C, transforms[1], dims[1])
+end
+function fft!(B::BlockedArray, A::BlockedArray, ws_AB::BlockedWorkspace, transforms, dims, ::Slab)
+    stage!(A, transforms[1], (dims[1], dims[2])); redistribute!(B, A, ws_AB)
+    return stage!(B, transforms[3], dims[3])
+end
+function ifft!(A::BlockedArray, B::BlockedArray, ws_BA::BlockedWorkspace, transforms, dims, ::Slab)
+    stage!(B, transforms[3], dims[3]); redistribute!(A, B, ws_BA)
+    return stage!(A, transforms[1], (dims[1], dims[2]))
 end
 
 # ---- batched transforms (dfft_batch_*; no reference counterpart) ----------
