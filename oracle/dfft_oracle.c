@@ -21,6 +21,8 @@
  *     (src/fft.jl:180-231) and of the GPU kernels pack_kernel!,
  *     unpack_kernel!, local_copy_kernel! (src/fftgpu.jl:452-495), 0-based.
  *   - the project's counter-based SplitMix64 input generator.
+ *   - spot bins of a 3D DFT by direct separable summation in long double,
+ *     the full-size check that needs no FFT (dfo_spot_bins).
  *
  * Arrays are column-major (Julia layout): element (i,j,k) of an n0 x n1 x n2
  * array sits at i + n0*(j + n1*k).  Complex data is interleaved (re, im).
@@ -445,6 +447,71 @@ void dfo_permute(void *out, const void *in, const int64_t *in_shape, const int *
                 int64_t di = idx[perm[0]] + on[0] * (idx[perm[1]] + on[1] * idx[perm[2]]);
                 memcpy(O + di * esize, I + si * esize, (size_t)esize);
             }
+}
+
+/* ------------------------------------------------------------------ */
+/* Spot bins of a 3D DFT by direct separable summation (long double)   */
+/* ------------------------------------------------------------------ */
+/* X[k0,k1,k2] = sum_{i,j,l} x[i,j,l] w0^(k0 i) w1^(k1 j) w2^(k2 l),
+ * w_d = exp(sign 2 pi i / n_d), for each of the nb bins (0-based index
+ * triples in `bins`).  The sum over i runs first for every (j, l), then
+ * over j, then over l: n0 n1 n2 + n1 n2 + n2 complex multiply-adds per bin,
+ * all in long double with long-double twiddles, so the reference value is
+ * independent of any FFT factorization and exact to about 1e-18 relative
+ * to sum |x|.  x: interleaved complex, column-major n0 x n1 x n2.
+ * out: nb interleaved complex results, rounded to double. */
+void dfo_spot_bins(const double *x, const int64_t *shape, const int64_t *bins, int64_t nb, int sign,
+                   double *out) {
+    const int64_t n0 = shape[0], n1 = shape[1], n2 = shape[2];
+    long double *c0 = (long double *)malloc(sizeof(long double) * 2 * (size_t)n0);
+    long double *c1 = (long double *)malloc(sizeof(long double) * 2 * (size_t)n1);
+    long double *c2 = (long double *)malloc(sizeof(long double) * 2 * (size_t)n2);
+    long double *a = (long double *)malloc(sizeof(long double) * 2 * (size_t)(n1 * n2));
+    for (int64_t b = 0; b < nb; ++b) {
+        const int64_t k0 = bins[3 * b], k1 = bins[3 * b + 1], k2 = bins[3 * b + 2];
+        /* w_d^(k m) = exp(sign 2 pi i (k m mod n) / n), reduced exactly in integers */
+        for (int64_t m = 0; m < n0; ++m) {
+            long double c, sn;
+            cis_ld((k0 * m) % n0, n0, &c, &sn);
+            c0[2 * m] = c; c0[2 * m + 1] = sign < 0 ? -sn : sn;
+        }
+        for (int64_t m = 0; m < n1; ++m) {
+            long double c, sn;
+            cis_ld((k1 * m) % n1, n1, &c, &sn);
+            c1[2 * m] = c; c1[2 * m + 1] = sign < 0 ? -sn : sn;
+        }
+        for (int64_t m = 0; m < n2; ++m) {
+            long double c, sn;
+            cis_ld((k2 * m) % n2, n2, &c, &sn);
+            c2[2 * m] = c; c2[2 * m + 1] = sign < 0 ? -sn : sn;
+        }
+        /* a[j,l] = sum_i x[i,j,l] w0^(k0 i) */
+#pragma omp parallel for schedule(static)
+        for (int64_t jl = 0; jl < n1 * n2; ++jl) {
+            const double *row = x + 2 * n0 * jl;
+            long double sr = 0, si = 0;
+            for (int64_t i = 0; i < n0; ++i) {
+                const long double xr = row[2 * i], xi = row[2 * i + 1];
+                sr += xr * c0[2 * i] - xi * c0[2 * i + 1];
+                si += xr * c0[2 * i + 1] + xi * c0[2 * i];
+            }
+            a[2 * jl] = sr; a[2 * jl + 1] = si;
+        }
+        long double tr = 0, ti = 0;
+        for (int64_t l = 0; l < n2; ++l) {
+            long double sr = 0, si = 0;
+            for (int64_t j = 0; j < n1; ++j) {
+                const long double ar = a[2 * (j + n1 * l)], ai = a[2 * (j + n1 * l) + 1];
+                sr += ar * c1[2 * j] - ai * c1[2 * j + 1];
+                si += ar * c1[2 * j + 1] + ai * c1[2 * j];
+            }
+            tr += sr * c2[2 * l] - si * c2[2 * l + 1];
+            ti += sr * c2[2 * l + 1] + si * c2[2 * l];
+        }
+        out[2 * b] = (double)tr;
+        out[2 * b + 1] = (double)ti;
+    }
+    free(c0); free(c1); free(c2); free(a);
 }
 
 int dfo_num_threads(void) {

@@ -78,6 +78,8 @@ def lib():
             L.dfo_unpack.argtypes = [P, P, I64P, I64P, I64P, I64, I64]
             L.dfo_local_copy.argtypes = [P, I64P, I64P, P, I64P, I64P, I64P, I64]
             L.dfo_permute.argtypes = [P, P, I64P, ctypes.POINTER(ctypes.c_int), I64]
+            L.dfo_spot_bins.argtypes = [P, I64P, I64P, I64, ctypes.c_int, P]
+            L.dfo_spot_bins.restype = None
             L.dfo_num_threads.restype = ctypes.c_int
             L.dfo_set_num_threads.argtypes = [ctypes.c_int]
             _lib = L
@@ -404,6 +406,19 @@ def r2r_logical_size(kind, n):
     if kind == DHT:
         return n
     return 2 * n
+
+
+def spot_bins(a, bins, sign=-1):
+    """Bins (k0, k1, k2) of the unnormalised 3D DFT of complex `a` (Julia
+    size n0 x n1 x n2) by direct separable summation in long double
+    (dfo_spot_bins): the reference value at sizes where a full oracle
+    transform is too slow."""
+    a = np.asfortranarray(a, dtype=np.complex128)
+    b = np.ascontiguousarray(np.asarray(bins, dtype=np.int64).reshape(-1, 3))
+    out = np.empty(len(b), dtype=np.complex128)
+    lib().dfo_spot_bins(_ptr(a), _shape3(a.shape), b.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(b),
+                        int(sign), _ptr(out))
+    return out
 
 
 def rel_l2(y, ref):

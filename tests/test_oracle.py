@@ -229,3 +229,27 @@ def test_checks_catch_planted_mistakes():
     O.c2c_axis(one, 0, -1, 1.0)
     O.c2c_axis(one, 1, -1, 1.0)
     assert O.rel_l2(one, ref) > 0.1
+
+
+# ---------------------------------------------------------------- spot bins
+@pytest.mark.parametrize("name", ["c2c_4x3x2", "c2c_8x6x5", "c2c_7x5x3"])
+def test_spot_bins_golden(gold, name):
+    """Every bin of the golden tiny transforms (mpmath brute force), by the
+    long-double separable summation; the wrong sign fails."""
+    a = gold[name + "_in"]
+    ref = gold[name + "_fwd"]
+    bins = [(i, j, k) for k in range(a.shape[2]) for j in range(a.shape[1]) for i in range(a.shape[0])]
+    got = O.spot_bins(a, bins)
+    assert O.rel_l2(got, np.array([ref[b] for b in bins])) < 1e-15
+    assert O.rel_l2(O.spot_bins(a, bins, sign=+1), np.array([ref[b] for b in bins])) > 1e-2
+
+
+def test_spot_bins_match_oracle_fft_mixed_sizes():
+    """Spot bins at 36x20x21 (factors 2, 3, 5, 7) against the oracle's FFT,
+    both signs, including the Nyquist and the last bins."""
+    a = O.fill(17, (36, 20, 21), np.complex128)
+    bins = [(0, 0, 0), (18, 10, 0), (35, 19, 20), (1, 0, 0), (0, 1, 0), (0, 0, 1), (7, 13, 5)]
+    F = O.fftn(a)
+    Fi = O.fftn(a, inverse=True) * a.size
+    assert O.rel_l2(O.spot_bins(a, bins), np.array([F[b] for b in bins])) < 1e-14
+    assert O.rel_l2(O.spot_bins(a, bins, sign=+1), np.array([Fi[b] for b in bins])) < 1e-14
