@@ -111,9 +111,26 @@ def cpu_baseline(n):
     t2 = time.perf_counter()
     N = float(n) ** 3
     t = ((t1 - t0) + (t2 - t1)) / 2
-    return {"value": round(5 * N * math.log2(N) * 1e-9 / t, 3), "unit": "GFLOP/s", "cores": O.num_threads(),
-            "kind": "port", "sample": "one full %d^3 ComplexF64 forward + inverse through the C oracle "
-            "(per-axis batched FFTs in the reference's stage order), time per transform %.3f s" % (n, t)}
+    out = {"value": round(5 * N * math.log2(N) * 1e-9 / t, 3), "unit": "GFLOP/s", "cores": O.num_threads(),
+           "kind": "port", "sample": "one full %d^3 ComplexF64 forward + inverse through the C oracle "
+           "(per-axis batched FFTs in the reference's stage order), time per transform %.3f s" % (n, t)}
+    # a library FFT on the same host cores beside it (pocketfft, the algorithm
+    # family of the reference's FFTW CPU path; FFTW itself is not installed)
+    try:
+        import scipy.fft as sf
+        workers = O.num_threads()  # the same cores as the oracle (the box's CPU share)
+        x = a.astype(np.complex128, copy=True)
+        t0 = time.perf_counter()
+        y = sf.fftn(x, workers=workers)
+        t1 = time.perf_counter()
+        sf.ifftn(y, workers=workers, overwrite_x=True)
+        t2 = time.perf_counter()
+        ts = ((t1 - t0) + (t2 - t1)) / 2
+        out["scipy_pocketfft"] = {"value": round(5 * N * math.log2(N) * 1e-9 / ts, 3), "unit": "GFLOP/s",
+                                  "workers": workers, "ms_per_transform": round(ts * 1e3, 1)}
+    except Exception as e:  # scipy absent: the oracle port stays the baseline
+        out["scipy_pocketfft"] = {"error": repr(e)}
+    return out
 
 
 def _time_events(fn, reps):

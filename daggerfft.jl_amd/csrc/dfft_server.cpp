@@ -248,11 +248,14 @@ void dfft_server::finish(Req *q, int status) {
     const bool ok = status == DFFT_OK;
     q->status = status;
     q->state = ok ? DFFT_REQ_DONE : DFFT_REQ_FAILED;
-    const void *keep = ok ? q->result : nullptr;
+    // DFFT_REQ_DISCARD: nobody reads the result, so its server buffer goes
+    // back to the arena now rather than at dfft_server_release
+    const bool discard = (q->r.req_flags & DFFT_REQ_DISCARD) != 0;
+    const void *keep = (ok && !discard) ? q->result : nullptr;
     if (q->in.p && q->in.p != keep) pool.release(q->in);
     if (q->work.p) pool.release(q->work);
     if (q->out.p && q->out.p != keep) pool.release(q->out);
-    if (!ok) q->result = nullptr;
+    if (!ok || (discard && !q->r.d_out)) q->result = nullptr;
     if (ok) ++completed;
     else ++failed;
     finished.push_back(q->ticket);
@@ -386,7 +389,7 @@ void dfft_server::loop() {
 
 static int validate(const dfft_request *r) {
     if (!r) return DFFT_ERR_ARG;
-    if (r->req_flags != 0) return DFFT_ERR_ARG;
+    if (r->req_flags & ~DFFT_REQ_DISCARD) return DFFT_ERR_ARG;
     if (r->kind != DFFT_C2C && r->kind != DFFT_R2C && r->kind != DFFT_C2R) return DFFT_ERR_UNSUPPORTED;
     if (r->sign != -1 && r->sign != 1) return DFFT_ERR_ARG;
     if (r->kind == DFFT_R2C && r->sign != -1) return DFFT_ERR_ARG;
@@ -601,6 +604,7 @@ int dfft_server_copy_result(dfft_server *s, int64_t ticket, void *d_dst, size_t 
         if (it == s->reqs.end()) return DFFT_ERR_ARG;
         const Req *q = it->second;
         if (q->state != DFFT_REQ_DONE) return q->state == DFFT_REQ_FAILED ? q->status : DFFT_ERR_NOT_READY;
+        if (!q->result) return DFFT_ERR_STATE;  // a discarded result
         src = q->result;
         nb = q->result_bytes;
     }

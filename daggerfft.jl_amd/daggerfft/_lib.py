@@ -81,6 +81,7 @@ class dfft_server_stats(ctypes.Structure):
 
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
+REQ_DISCARD = 0x1
 
 
 def _declare(L):

@@ -113,7 +113,7 @@ def serving_trace(count=2000, seed=8, rate=200.0, lo=64, hi=512, c2c_frac=0.6, f
     return out
 
 
-def _request_struct(req, t_arrival=0.0, d_in=None, d_out=None, decomp=L.PENCIL, flags=0):
+def _request_struct(req, t_arrival=0.0, d_in=None, d_out=None, decomp=L.PENCIL, flags=0, req_flags=0):
     r = L.dfft_request()
     r.n[0], r.n[1], r.n[2] = [int(s) for s in req.shape]
     r.kind = req.kind
@@ -121,7 +121,7 @@ def _request_struct(req, t_arrival=0.0, d_in=None, d_out=None, decomp=L.PENCIL, 
     r.precision = req.precision
     r.decomp = decomp
     r.flags = flags
-    r.req_flags = 0
+    r.req_flags = req_flags
     r.seed = req.seed
     r.d_in = d_in
     r.d_out = d_out
@@ -154,11 +154,13 @@ class Server:
         arr = (L.dfft_request * max(len(reqs), 1))(*[_request_struct(q) for q in reqs])
         check(lib().dfft_server_prepare(self._h, arr, len(reqs)), "dfft_server_prepare")
 
-    def submit(self, req, t_arrival=0.0, inp=None, out=None):
+    def submit(self, req, t_arrival=0.0, inp=None, out=None, discard=False):
         """Queue `req`; returns its ticket.  inp/out: optional DeviceArrays the
-        request reads / writes instead of server buffers (kept alive by the caller)."""
+        request reads / writes instead of server buffers (kept alive by the
+        caller).  discard: the result is not read (DFFT_REQ_DISCARD): its
+        server buffer returns to the arena when the request completes."""
         r = _request_struct(req, t_arrival, inp.ptr if inp is not None else None,
-                            out.ptr if out is not None else None)
+                            out.ptr if out is not None else None, req_flags=L.REQ_DISCARD if discard else 0)
         t = ctypes.c_int64()
         check(lib().dfft_server_submit(self._h, ctypes.byref(r), ctypes.byref(t)), "dfft_server_submit")
         return t.value
@@ -299,7 +301,9 @@ def replay(server, trace, open_loop=True, keep=None, time_scale=1.0):
                         break
                     if due - now > 0.002:
                         time.sleep(due - now - 0.001)
-            t = server.submit(q, t_arrival=due if open_loop else 0.0)
+            # results outside `keep` are never read: the server frees them at
+            # completion instead of waiting for the reaper's release
+            t = server.submit(q, t_arrival=due if open_loop else 0.0, discard=r not in keep)
             with lock:
                 index[t] = r
             submitted += 1

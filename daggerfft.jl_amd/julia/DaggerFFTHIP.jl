@@ -412,13 +412,17 @@ function Server(; hbm_budget=200 * 10^9, max_inflight=32, lanes=4)
     return s
 end
 
+const DFFT_REQ_DISCARD = UInt32(0x1)
+
 # fft (complex input) or rfft (real input) of a 3D array, or ifft with sign=+1;
 # with `out`, the result is written there, else it stays in the server until
-# release.  Returns the ticket.
-function submit(s::Server, A::ROCArray{T,3}; out=nothing, sign=-1, decomp=Pencil()) where {T}
+# release (discard=true: nobody reads it, and its server buffer is freed when
+# the request completes).  Returns the ticket.
+function submit(s::Server, A::ROCArray{T,3}; out=nothing, sign=-1, decomp=Pencil(), discard=false) where {T}
     kind = T <: Real ? DFFT_R2C : DFFT_C2C
     req = Ref(Request(Int64.(size(A)), Cint(kind), Cint(sign), Cint(prec_code(T)), Cint(decomp_code(decomp)),
-                      UInt32(0), UInt32(0), UInt64(0), pointer(A), out === nothing ? C_NULL : pointer(out), 0.0))
+                      UInt32(0), discard ? DFFT_REQ_DISCARD : UInt32(0), UInt64(0), pointer(A),
+                      out === nothing ? C_NULL : pointer(out), 0.0))
     t = Ref{Int64}(0)
     check(ccall((:dfft_server_submit, libdfft), Cint, (Ptr{Cvoid}, Ptr{Request}, Ptr{Int64}), s.handle, req, t))
     return t[]

@@ -287,13 +287,21 @@ typedef struct {
     int precision;          /* DFFT_F32 / DFFT_F64 */
     int decomp;             /* DFFT_PENCIL / DFFT_SLAB */
     uint32_t flags;         /* plan flags (DFFT_FLAG_UNNORMALIZED, DFFT_FLAG_NO_TUNE, ...) */
-    uint32_t req_flags;     /* reserved, must be 0 */
+    uint32_t req_flags;     /* 0 or DFFT_REQ_DISCARD */
     uint64_t seed;          /* d_in == NULL: the input is dfft_fill_uniform(seed) of the input shape */
     const void *d_in;       /* the caller's device input, read and left untouched; or NULL */
     void *d_out;            /* the caller's device output; or NULL: the result stays in a server
                                buffer until dfft_server_release */
     double t_arrival;       /* arrival time on the server clock (dfft_server_clock); 0: time of submission */
 } dfft_request;
+
+/* req_flags: the result is not read.  The server returns a server-held
+ * output buffer to its arena as soon as the request completes (a caller's
+ * d_out is written as usual); the request's record (state, times) stays until
+ * dfft_server_release, and its d_result reads NULL.  For callers that keep no
+ * results, such as throughput runs, so finished results do not hold memory
+ * that queued requests are waiting for. */
+#define DFFT_REQ_DISCARD  0x1u
 
 #define DFFT_REQ_QUEUED   0
 #define DFFT_REQ_RUNNING  1

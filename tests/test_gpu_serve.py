@@ -172,3 +172,34 @@ def test_cancel_queued_request(gpu):
         st = srv.stats()
     assert st["completed"] == 2 and st["queued"] == 0
     assert O.rel_l2(y2, _alone(q[2])) <= 1e-12
+
+
+def test_discarded_results_free_memory_at_completion(gpu):
+    """DFFT_REQ_DISCARD: with room for one request's buffer, discarded
+    requests run one after another without any release (their buffers return
+    to the arena when each completes); their records report no result, a copy
+    of it is refused, and a kept request after them is still correct.
+    Unknown request flags are rejected."""
+    import ctypes
+
+    from daggerfft import serve as S
+    from daggerfft import _lib as L
+    q = [S.Request((64, 64, 64), L.C2C, L.F64, 30 + i) for i in range(4)]  # 4 MiB each
+    with S.Server(hbm_budget=6 << 20, max_inflight=4, lanes=1) as srv:
+        ts = [srv.submit(x, discard=True) for x in q[:3]]
+        for t in ts:
+            assert srv.wait(t, 60.0)
+            inf = srv.info(t)
+            assert inf["state"] == L.REQ_DONE and inf["result_ptr"] == 0
+        assert srv.stats()["bytes_busy"] == 0
+        with pytest.raises(L.DFFTError):
+            srv.result(ts[0], q[0])
+        tk = srv.submit(q[3])
+        assert srv.wait(tk, 60.0)
+        y = srv.result(tk, q[3]).to_host()
+        for t in ts + [tk]:
+            srv.release(t)
+        r = S._request_struct(q[0], req_flags=0x2)
+        t = ctypes.c_int64()
+        assert L.lib().dfft_server_submit(srv._h, ctypes.byref(r), ctypes.byref(t)) == L.DFFT_ERR_ARG
+    assert O.rel_l2(y, _alone(q[3])) <= 1e-12
