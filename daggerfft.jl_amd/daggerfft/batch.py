@@ -7,8 +7,6 @@ from __future__ import annotations
 
 import ctypes
 
-import numpy as np
-
 from . import _lib as L
 from ._lib import check, lib
 
