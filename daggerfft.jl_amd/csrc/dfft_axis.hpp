@@ -77,6 +77,12 @@ struct TileCtx {
         return reinterpret_cast<const cx<T> *>(a.tw) + a.sp.post;
     }
 
+    // what the runtime stages read: FFT length, threads per line, twiddle table
+    typedef T value_type;
+    __device__ __forceinline__ int len() const { return a.sp.L; }
+    __device__ __forceinline__ int tpl() const { return a.TPL; }
+    __device__ __forceinline__ const cx<T> *twb() const { return reinterpret_cast<const cx<T> *>(a.tw); }
+
     // value entering the FFT at position p (stage 0)
     __device__ __forceinline__ cx<T> load0(int p) const {
         const T *inr = reinterpret_cast<const T *>(in);
@@ -213,17 +219,21 @@ struct TileCtx {
 };
 
 // Stage position in the schedule: where its input comes from and where its
-// output goes.
-enum StageKind : int { SK_FIRST = 0, SK_MID = 1, SK_LAST = 2, SK_ONLY = 3 };
+// output goes.  SK_FIRST_LDS is a first stage (no twiddles) whose input is
+// already in LDS in natural order (the second FFT of a Bluestein pass).
+enum StageKind : int { SK_FIRST = 0, SK_MID = 1, SK_LAST = 2, SK_ONLY = 3, SK_FIRST_LDS = 4 };
 
 // One Stockham stage with compile-time radix R.  Each thread owns up to
-// EMAX/R butterflies j = t, t+TPL, ...
-template <int R, int EMAX, int SK, class T, int OP, bool CONTIG>
-__device__ __forceinline__ void run_stage(const TileCtx<T, OP, CONTIG> &c, int Ns, int twoff) {
-    constexpr bool first = SK == SK_FIRST || SK == SK_ONLY;
+// EMAX/R butterflies j = t, t+TPL, ...  C is the tile context: its load0 and
+// store_last are the fused global maps of the first and last stage.
+template <int R, int EMAX, int SK, class C>
+__device__ __forceinline__ void run_stage(const C &c, int Ns, int twoff) {
+    typedef typename C::value_type T;
+    constexpr bool global_in = SK == SK_FIRST || SK == SK_ONLY;
+    constexpr bool first = global_in || SK == SK_FIRST_LDS;
     constexpr bool last = SK == SK_LAST || SK == SK_ONLY;
     constexpr int NBM = EMAX / R;
-    const int L = c.a.sp.L, NB = L / R, TPL = c.a.TPL;
+    const int L = c.len(), NB = L / R, TPL = c.tpl();
     cx<T> v[NBM][R];
 #pragma unroll
     for (int b = 0; b < NBM; ++b) {
@@ -231,20 +241,20 @@ __device__ __forceinline__ void run_stage(const TileCtx<T, OP, CONTIG> &c, int N
         if (j < NB) {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                if constexpr (first) v[b][r] = c.load0(j + r * NB);
+                if constexpr (global_in) v[b][r] = c.load0(j + r * NB);
                 else v[b][r] = c.lds[c.li(j + r * NB)];
             }
         }
     }
     // every read of the shared slots must finish before anyone overwrites them
-    if constexpr (!first) __syncthreads();
+    if constexpr (!global_in) __syncthreads();
 #pragma unroll
     for (int b = 0; b < NBM; ++b) {
         const int j = c.t + b * TPL;
         if (j < NB) {
             if constexpr (!first) {
                 const int k = j % Ns;
-                const cx<T> *w = reinterpret_cast<const cx<T> *>(c.a.tw) + twoff + k * (R - 1);
+                const cx<T> *w = c.twb() + twoff + k * (R - 1);
 #pragma unroll
                 for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], w[r - 1]);
             }
@@ -263,8 +273,8 @@ __device__ __forceinline__ void run_stage(const TileCtx<T, OP, CONTIG> &c, int N
     if constexpr (!last) __syncthreads();
 }
 
-template <int SK, int EMAX, class T, int OP, bool CONTIG>
-__device__ __forceinline__ void dispatch_stage(const TileCtx<T, OP, CONTIG> &c, int R, int Ns, int twoff) {
+template <int SK, int EMAX, class C>
+__device__ __forceinline__ void dispatch_stage(const C &c, int R, int Ns, int twoff) {
     switch (R) {
     case 8: run_stage<8, EMAX, SK>(c, Ns, twoff); break;
     case 4: run_stage<4, EMAX, SK>(c, Ns, twoff); break;
@@ -576,13 +586,12 @@ __device__ __forceinline__ int64_t pass_tile(const AxisPass &a) {
     return a.swz > 1 ? (int64_t)((b * (uint64_t)a.swz) % (uint64_t)gridDim.x) : (int64_t)b;
 }
 
-template <int EMAX, class T, int OP, bool CONTIG, bool SPLIT = false, class RL = NoRadices, int TPLC = 0>
-__device__ __forceinline__ void axis_tile(const AxisPass &a, const void *in, void *out, int64_t tile, int tid,
-                                          cx<T> *lds) {
-    constexpr bool FIXED = !std::is_same<RL, NoRadices>::value;
-    const int TPL = FIXED ? TPLC : a.TPL, TC = a.TC;
+// Thread -> (line slot ls, thread-in-line t) and the tile's global bases.
+template <bool CONTIG>
+__device__ __forceinline__ void tile_map(const AxisPass &a, int64_t tile, int tid, int TPL, int &ls, int &t,
+                                         bool &valid, int64_t &ib, int64_t &ob) {
+    const int TC = a.TC;
     const bool active = tid < TPL * TC;
-    int ls, t;
     if constexpr (CONTIG) {
         ls = tid / TPL;
         t = tid - ls * TPL;
@@ -594,8 +603,6 @@ __device__ __forceinline__ void axis_tile(const AxisPass &a, const void *in, voi
         ls = 0;
         t = 1 << 28;  // beyond every butterfly / bin index: skips all work
     }
-    bool valid;
-    int64_t ib, ob;
     if constexpr (CONTIG) {
         const int64_t line = tile * TC + ls;
         valid = active && line < a.nlines;
@@ -618,9 +625,38 @@ __device__ __forceinline__ void axis_tile(const AxisPass &a, const void *in, voi
         ib = col + o * a.in_so;
         ob = col + o * a.out_so;
     }
+}
+
+// R2C post-processing of the H-point FFT Z that the last stage left in LDS
+// (natural order): X_k = 1/2 [(Z_k + conj Z_{H-k}) - i w^k (Z_k - conj Z_{H-k})],
+// k = 0..H.
+template <class T, bool CONTIG>
+__device__ __forceinline__ void r2c_post_lds(const TileCtx<T, OP_R2C, CONTIG> &c, int TPL) {
+    __syncthreads();
+    const int H = c.a.sp.L;
+    const T s = (T)c.a.scale;
+    cx<T> *outc = reinterpret_cast<cx<T> *>(c.out);
+    const cx<T> *pw = c.post();
+    for (int k = c.t; k <= H; k += TPL) {
+        cx<T> zk = c.lds[c.li(k == H ? 0 : k)];
+        cx<T> zc = conj(c.lds[c.li(k == 0 ? 0 : H - k)]);
+        cx<T> A = zk + zc, B = zk - zc;
+        cx<T> X = scl(A + mul_neg_i(cmul(pw[k], B)), (T)0.5 * s);
+        if (c.valid) gstore(outc + c.ob + (int64_t)k * c.a.out_sm, X, c.a.nt);
+    }
+}
+
+template <int EMAX, class T, int OP, bool CONTIG, bool SPLIT = false, class RL = NoRadices, int TPLC = 0>
+__device__ __forceinline__ void axis_tile(const AxisPass &a, const void *in, void *out, int64_t tile, int tid,
+                                          cx<T> *lds) {
+    constexpr bool FIXED = !std::is_same<RL, NoRadices>::value;
+    const int TPL = FIXED ? TPLC : a.TPL;
+    int ls, t;
+    bool valid;
+    int64_t ib, ob;
+    tile_map<CONTIG>(a, tile, tid, TPL, ls, t, valid, ib, ob);
     TileCtx<T, OP, CONTIG> c{a, in, out, lds, ls, t, valid, ib, ob};
     const StagePlan &sp = a.sp;
-
 
     if (sp.S == 0) {  // length-1 transform: only the fused load/store maps apply
         if (t == 0) c.store_last(0, c.load0(0));
@@ -639,21 +675,73 @@ __device__ __forceinline__ void axis_tile(const AxisPass &a, const void *in, voi
         dispatch_stage<SK_LAST, EMAX>(c, sp.R[sp.S - 1], Ns, sp.tw[sp.S - 1]);
     }
 
-    if constexpr (OP == OP_R2C && !(FIXED && SPLIT)) {
-        // X_k = 1/2 [(Z_k + conj Z_{H-k}) - i w^k (Z_k - conj Z_{H-k})], k = 0..H
-        __syncthreads();
-        const int H = sp.L;
-        const T s = (T)a.scale;
-        cx<T> *outc = reinterpret_cast<cx<T> *>(out);
-        const cx<T> *pw = c.post();
-        for (int k = t; k <= H; k += TPL) {
-            cx<T> zk = lds[c.li(k == H ? 0 : k)];
-            cx<T> zc = conj(lds[c.li(k == 0 ? 0 : H - k)]);
-            cx<T> A = zk + zc, B = zk - zc;
-            cx<T> X = scl(A + mul_neg_i(cmul(pw[k], B)), (T)0.5 * s);
-            if (valid) gstore(outc + ob + (int64_t)k * a.out_sm, X, a.nt);
+    if constexpr (OP == OP_R2C && !(FIXED && SPLIT)) r2c_post_lds(c, TPL);
+}
+
+// ---------------------------------------------------------------------------
+// Bluestein (chirp-z) tiles, for line lengths with a prime factor > 7
+// (Bluestein 1970; Rabiner, Schafer & Rader 1969).  With c_m = exp(-i pi m^2/n)
+// and jk = (j^2 + k^2 - (k-j)^2)/2, the length-n DFT the op's maps ask for is
+//   X_k = c_k sum_j (x_j c_j) conj(c_{k-j}),
+// a cyclic convolution of length M >= 2n-1 with b_m = conj(c_m) = b_{M-m}.
+// Phase 0 is an M-point FFT of a_j = x_j c_j (zero for j >= n) whose last
+// stage leaves conj(A_p bhat_p) in LDS, bhat = FFT_M(b) / M.  Phase 1 is an
+// M-point FFT of that; the conjugate of its output is the convolution
+// (IFFT_M(y) = conj(FFT_M(conj y)) / M), and c_k times it is X_k for k < n.
+// The op's own load0/store_last give x_j and take X_k, so every fused op
+// (R2C/C2R packing, the r2r kinds) runs unchanged around the chirp-z core.
+// ---------------------------------------------------------------------------
+template <class T, int OP, bool CONTIG, int PHASE>
+struct BlueCtx : TileCtx<T, OP, CONTIG> {
+    typedef TileCtx<T, OP, CONTIG> Op;  // the op's maps; a.sp.L = n
+    // Loaded once: reading them through `a` inside the stage code lets the
+    // compiler merge the radix cases' tails over pointers into the kernel
+    // argument, which then gets copied to scratch.
+    const cx<T> *chirp, *bhat;
+    int n, M;
+
+    __device__ __forceinline__ BlueCtx(const Op &c)
+        : Op(c), chirp(c.twb() + c.a.chirp), bhat(c.twb() + c.a.bhat), n(c.a.sp.L), M(c.a.bsp.L) {}
+    __device__ __forceinline__ int len() const { return M; }
+
+    __device__ __forceinline__ cx<T> load0(int p) const {  // phase 0 only
+        if (p >= n) return cx<T>{0, 0};
+        return cmul(Op::load0(p), chirp[p]);
+    }
+    __device__ __forceinline__ void store_last(int p, cx<T> v) const {
+        if constexpr (PHASE == 0) {
+            this->lds[this->li(p)] = conj(cmul(v, bhat[p]));
+        } else {
+            if (p < n) Op::store_last(p, cmul(conj(v), chirp[p]));
         }
     }
+};
+
+template <int EMAX, bool LDS_IN, class C>
+__device__ __forceinline__ void blue_fft(const C &c, const StagePlan &bp) {
+    // the planner gives M >= 2n-1 >= 21, so at least two stages
+    dispatch_stage<LDS_IN ? SK_FIRST_LDS : SK_FIRST, EMAX>(c, bp.R[0], 1, bp.tw[0]);
+    int Ns = bp.R[0];
+    for (int s = 1; s < bp.S - 1; ++s) {
+        dispatch_stage<SK_MID, EMAX>(c, bp.R[s], Ns, bp.tw[s]);
+        Ns *= bp.R[s];
+    }
+    dispatch_stage<SK_LAST, EMAX>(c, bp.R[bp.S - 1], Ns, bp.tw[bp.S - 1]);
+}
+
+template <int EMAX, class T, int OP, bool CONTIG>
+__device__ __forceinline__ void blue_tile(const AxisPass &a, const void *in, void *out, int64_t tile, int tid,
+                                          cx<T> *lds) {
+    int ls, t;
+    bool valid;
+    int64_t ib, ob;
+    tile_map<CONTIG>(a, tile, tid, a.TPL, ls, t, valid, ib, ob);
+    TileCtx<T, OP, CONTIG> c{a, in, out, lds, ls, t, valid, ib, ob};
+    blue_fft<EMAX, false>(BlueCtx<T, OP, CONTIG, 0>(c), a.bsp);
+    // phase 0's last stage wrote LDS after its own barrier; phase 1 reads it
+    __syncthreads();
+    blue_fft<EMAX, true>(BlueCtx<T, OP, CONTIG, 1>(c), a.bsp);
+    if constexpr (OP == OP_R2C) r2c_post_lds(c, a.TPL);
 }
 
 }  // namespace dfft

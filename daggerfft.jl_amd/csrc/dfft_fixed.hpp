@@ -41,6 +41,9 @@ struct FixedTable {
     int n;
 };
 
+// the Bluestein kernel of (precision, layout, op), for AxisPass::blue passes (dfft_blue.hip)
+axis_kernel_t blue_kernel(int prec, int contig, int op);
+
 #define DFFT_FX(T, P, OPV, CONTIG, VAR, EMAXV, SPLIT, RL)                                             \
     {P, OPV, CONTIG, VAR, RL::L, fixed_tpl<RL, EMAXV>(), RL::S, RL::R,                                 \
      (axis_kernel_t)k_axis_fixed<T, OPV, CONTIG, EMAXV, SPLIT, RL, fixed_tpl<RL, EMAXV>()>}

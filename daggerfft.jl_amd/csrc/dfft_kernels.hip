@@ -1,6 +1,7 @@
 // dfft_kernels.hip -- HIP kernels of libdaggerfft_hip for gfx950.
 //
-//   k_axis      one Stockham axis pass per launch (C2C / R2C / C2R / every r2r kind)
+//   k_axis      one Stockham axis pass per launch (C2C / R2C / C2R / every r2r kind);
+//               catalog kernels in dfft_fixed_*.hip, Bluestein kernels in dfft_blue.hip
 //   k_fill      SplitMix64 counter-based uniform[0,1) input generator
 //   k_box       box gather / scatter / copy (pack_kernel!, unpack_kernel!,
 //               local_copy_kernel!; src/fftgpu.jl:452-495), coalesced along axis 0
@@ -90,6 +91,7 @@ std::vector<std::vector<int>> fixed_lists(int prec, int contig, int variant, int
 }
 
 static axis_kernel_t axis_kernel(const AxisPass &a, int prec) {
+    if (a.blue) return blue_kernel(prec, a.contig, a.op);
     if (a.variant >= 1) {
         const FixedEntry *e = fixed_entry(prec, a.contig, a.variant, a.op, a.sp.L, a.sp.R, a.sp.S);
         return e ? e->fn : nullptr;

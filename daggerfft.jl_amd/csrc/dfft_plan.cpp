@@ -111,15 +111,64 @@ static void build_twiddles(StagePlan &sp, int op, int nlog, std::vector<long dou
     (void)nlog;
 }
 
+// Bluestein blocks of a length-n pass convolving over M = ap.bsp.L: the
+// stage twiddles of M, the chirp c_p = exp(-i pi p^2 / n) = w_{2n}^{p^2 mod 2n}
+// (p < n; the exponent reduced exactly in integers), and bhat = FFT_M(b) / M
+// for b_m = conj(c_m) at m and M-m (|m| < n), zero elsewhere.  b is even, so
+// bhat_k = (b_0 + 2 sum_{m=1}^{n-1} b_m cos(2 pi m k / M)) / M, summed in long
+// double.
+static void build_bluestein(AxisPass &ap, std::vector<long double> &tab) {
+    const int n = ap.sp.L, M = ap.bsp.L;
+    build_twiddles(ap.bsp, OP_C2C, M, tab);
+    ap.chirp = (int)(tab.size() / 2);
+    std::vector<long double> br(n), bi(n);
+    for (int p = 0; p < n; ++p) {
+        const int64_t e = ((int64_t)p * p) % (2 * (int64_t)n);
+        long double c, sn;
+        cis_ld(e, 2 * (int64_t)n, &c, &sn);  // c_p = c - i sn
+        tab.push_back(c);
+        tab.push_back(-sn);
+        br[p] = c;  // b_p = conj(c_p) = c + i sn
+        bi[p] = sn;
+    }
+    std::vector<long double> cosM(M);
+    for (int k = 0; k < M; ++k) {
+        long double c, sn;
+        cis_ld(k, M, &c, &sn);
+        cosM[k] = c;
+    }
+    ap.bhat = (int)(tab.size() / 2);
+    for (int k = 0; k < M; ++k) {
+        long double re = br[0], im = bi[0];
+        int64_t mk = 0;  // m*k mod M
+        for (int m = 1; m < n; ++m) {
+            mk += k;
+            if (mk >= M) mk -= M;
+            re += 2 * br[m] * cosM[mk];
+            im += 2 * bi[m] * cosM[mk];
+        }
+        tab.push_back(re / M);
+        tab.push_back(im / M);
+    }
+}
+
 void TwiddleTable::add(AxisPass &ap) {
     std::string key = std::to_string(ap.op) + ":" + std::to_string(ap.sp.L);
     for (int s = 0; s < ap.sp.S; ++s) key += "," + std::to_string(ap.sp.R[s]);
+    if (ap.blue) {
+        key += "/B" + std::to_string(ap.bsp.L);
+        for (int s = 0; s < ap.bsp.S; ++s) key += "," + std::to_string(ap.bsp.R[s]);
+    }
     auto it = seen.find(key);
     if (it != seen.end()) {
-        ap.sp = it->second;
+        ap.sp = it->second.sp;
+        ap.bsp = it->second.bsp;
+        ap.chirp = it->second.chirp;
+        ap.bhat = it->second.bhat;
     } else {
         build_twiddles(ap.sp, ap.op, ap.nlog, tab);
-        seen[key] = ap.sp;
+        if (ap.blue) build_bluestein(ap, tab);
+        seen[key] = TwiddleBlocks{ap.sp, ap.bsp, ap.chirp, ap.bhat};
     }
 }
 
@@ -169,7 +218,25 @@ static bool tpl_ok(int L, const std::vector<int> &R, int emax, int tpl) {
 int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_lds, int contig_threads,
                   bool grouped) {
     AxisPass &a = pp.ap;
-    const int L = a.sp.L;
+    // A length with a prime factor > 7 runs Bluestein's convolution over the
+    // smallest 7-smooth M >= 2n-1 (runtime engine; the grouped kernel has no
+    // Bluestein tile).
+    a.blue = 0;
+    a.bsp = StagePlan{};
+    {
+        std::vector<int> f;
+        if (!factorize(a.sp.L, f)) {
+            if (grouped) return DFFT_ERR_UNSUPPORTED_SIZE;
+            int M = 2 * a.sp.L - 1;
+            while (!factorize(M, f)) ++M;
+            a.blue = 1;
+            a.bsp.L = M;
+            a.sp.S = 0;
+            tune = nullptr;
+        }
+    }
+    StagePlan &fp = a.blue ? a.bsp : a.sp;  // the plan of the FFT the tile runs
+    const int L = fp.L;
     const int es = pp.prec ? 16 : 8;
     // variants 1-3: a compile-time schedule of this length (see dfft_fixed.hpp).
     // Default (measured on gfx950): the split-LDS schedule wherever the
@@ -178,7 +245,8 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
     // powers of two, where 8 values per thread is as fast; then 16 values.
     std::vector<int> prefs;
     const bool pow2 = (L & (L - 1)) == 0;
-    if (tune) prefs = {tune->variant};
+    if (a.blue) prefs = {};
+    else if (tune) prefs = {tune->variant};
     else if (a.contig && pow2) prefs = {1, 2, 3};
     else prefs = {2, 1, 3};
     if (tune && tune->variant >= 2) prefs.push_back(1);
@@ -204,8 +272,8 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
     int prod = 1;
     for (int r : R) prod *= r;
     if (prod != L || (int)R.size() > MAX_STAGES) return DFFT_ERR_UNSUPPORTED_SIZE;
-    a.sp.S = (int)R.size();
-    for (int s = 0; s < a.sp.S; ++s) a.sp.R[s] = R[s];
+    fp.S = (int)R.size();
+    for (int s = 0; s < fp.S; ++s) fp.R[s] = R[s];
     const int emax = a.variant >= 2 ? 16 : EMAX;
     const int lds_es = a.variant == 2 ? es / 2 : es;
     int tpl = a.variant ? fixed_tpl
@@ -483,8 +551,8 @@ int build_plan(dfft_plan *p) {
                          p->ndims == 3 && n[2] > 1 &&
                          p->dims[2] == 2 && p->kind != DFFT_R2R && !(p->flags & DFFT_FLAG_GRAPH) &&
                          std::all_of(p->passes.begin(), p->passes.end(), [](const PassPlan &q) {
-                             // ops the grouped kernel dispatches
-                             return q.ap.op == OP_C2C || q.ap.op == OP_R2C || q.ap.op == OP_C2R;
+                             // ops the grouped kernel dispatches (it has no Bluestein tile)
+                             return (q.ap.op == OP_C2C || q.ap.op == OP_R2C || q.ap.op == OP_C2R) && !q.ap.blue;
                          });
     // chunk: planes whose dims[0]-pass output is about `chunk_bytes`
     int64_t chunk_bytes = 32LL << 20;
@@ -794,16 +862,20 @@ int dfft_plan_describe(const dfft_plan *p, int pass, char *buf, size_t buflen) {
     if (!p || !buf || pass < 0 || pass >= (int)p->passes.size()) return DFFT_ERR_ARG;
     const PassPlan &pp = p->passes[pass];
     const AxisPass &a = pp.ap;
+    // radices of the FFT the tile runs (a Bluestein pass: of its convolution length)
+    const StagePlan &fp = a.blue ? a.bsp : a.sp;
     std::string rad, rl;
-    for (int s = 0; s < a.sp.S; ++s) {
-        rad += (s ? "x" : "") + std::to_string(a.sp.R[s]);
-        rl += (s ? "," : "") + std::to_string(a.sp.R[s]);
+    for (int s = 0; s < fp.S; ++s) {
+        rad += (s ? "x" : "") + std::to_string(fp.R[s]);
+        rl += (s ? "," : "") + std::to_string(fp.R[s]);
     }
     if (rad.empty()) rad = "1";
     // the kernel's name as profilers print it (spaces removed)
     const char *T = pp.prec ? "double" : "float";
     std::string kname;
-    if (a.variant >= 1)
+    if (a.blue)
+        kname = std::string("k_axis_blue<") + T + "," + std::to_string(a.op) + "," + (a.contig ? "true" : "false") + ">";
+    else if (a.variant >= 1)
         kname = std::string("k_axis_fixed<") + T + "," + std::to_string(a.op) + "," + (a.contig ? "true" : "false") +
                 "," + (a.variant == 2 ? "16,true" : a.variant == 3 ? "16,false" : "8,false") + ",dfft::Radices<" + rl +
                 ">," + std::to_string(a.TPL) + ">";
@@ -812,9 +884,10 @@ int dfft_plan_describe(const dfft_plan *p, int pass, char *buf, size_t buflen) {
     snprintf(buf, buflen,
              "{\"pass\":%d,\"axis\":%d,\"op\":\"%s\",\"L\":%d,\"nlog\":%d,\"radices\":\"%s\",\"TPL\":%d,\"TC\":%d,"
              "\"contig\":%d,\"variant\":%d,\"threads\":%d,\"blocks\":%lld,\"lds\":%zu,\"bytes\":%.0f,\"prec\":\"%s\","
-             "\"swz\":%d,\"kernel\":\"%s\"}",
+             "\"swz\":%d,\"bluestein\":%d,\"kernel\":\"%s\"}",
              pass, pp.axis, op_name(a.op), a.sp.L, a.nlog, rad.c_str(), a.TPL, a.TC, a.contig, a.variant, pp.threads,
-             (long long)pp.nblocks, pp.lds, pp.alg_bytes, pp.prec ? "f64" : "f32", a.swz, kname.c_str());
+             (long long)pp.nblocks, pp.lds, pp.alg_bytes, pp.prec ? "f64" : "f32", a.swz, a.blue ? a.bsp.L : 0,
+             kname.c_str());
     return DFFT_OK;
 }
 

@@ -48,13 +48,19 @@ void set_geometry(AxisPass &a, int axis, const int64_t nin[3], const int64_t nou
 int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads = 1024, size_t max_lds = 0,
                   int contig_threads = 256, bool grouped = false);
 
-// Stage twiddle + pre/post blocks of every distinct (op, L, radices) of one
-// precision, in one device table.
+// Offsets of one pass's blocks in a TwiddleTable.
+struct TwiddleBlocks {
+    StagePlan sp, bsp;
+    int chirp = 0, bhat = 0;
+};
+
+// Stage twiddle + pre/post blocks (+ Bluestein blocks) of every distinct
+// (op, L, radices) of one precision, in one device table.
 struct TwiddleTable {
     int prec = 1;
     std::vector<long double> tab;           // (re, im) pairs
-    std::map<std::string, StagePlan> seen;
-    void add(AxisPass &ap);                 // sets ap.sp.tw / ap.sp.post offsets
+    std::map<std::string, TwiddleBlocks> seen;
+    void add(AxisPass &ap);                 // sets the offsets in ap.sp (and ap.bsp, chirp, bhat)
     int upload(void **d_ptr, size_t *bytes) const;
 };
 bool tuning_lookup(const PassPlan &pp, Tuning &t);

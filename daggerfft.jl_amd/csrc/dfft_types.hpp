@@ -66,6 +66,14 @@ struct AxisPass {
     int64_t nouter;           // strided: number of outer slices (1 for contig)
     int64_t in_sm, in_so;     // input: stride along the line, stride between lines/outer
     int64_t out_sm, out_so;   // output: same
+    // Bluestein (chirp-z) pass: a length sp.L with a prime factor > 7 runs its
+    // DFT as a cyclic convolution of length bsp.L >= 2*sp.L - 1, through two
+    // bsp.L-point Stockham FFTs in LDS (sp then holds no stages, only sp.L
+    // and the op's pre/post block).  0: an ordinary pass.
+    int blue;
+    int chirp;                // offset of c_p = exp(-i pi p^2 / sp.L), p < sp.L
+    int bhat;                 // offset of FFT_M(b)_k / M, b_m = conj(c_m) = b_{M-m}, M = bsp.L
+    StagePlan bsp;            // schedule of the convolution length M
 };
 
 }  // namespace dfft

@@ -1,0 +1,158 @@
+"""Bluestein (chirp-z) passes: line lengths with a prime factor above 7,
+which FFTW and cuFFT accept and the Stockham catalog cannot factor.  Each
+such axis runs as two M-point FFTs of a cyclic convolution in LDS (M >= 2n-1,
+7-smooth), around the same fused load/store maps as every other pass.
+
+Parity with the oracle (whose recursive FFT ends in direct DFTs of the prime
+factors, pinned against scipy in test_oracle.py): ComplexF64/Float64 rel-L2
+<= 1e-12; ComplexF32/Float32 against the FP64 oracle <= 2e-5."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+TOL64 = 1e-12
+TOL32 = 2e-5
+
+
+def _dev(a):
+    import daggerfft as D
+    return D.DeviceArray.from_host(a)
+
+
+C2C_SHAPES = [(11, 13, 17), (97, 6, 5), (8, 101, 4), (6, 5, 127), (1009, 4, 3), (4, 3, 1031), (22, 26, 34),
+              (2003, 2, 2), (2, 4093, 1), (4093, 2, 1)]
+
+
+@pytest.mark.parametrize("shape", C2C_SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("prec", [1, 0], ids=["f64", "f32"])
+def test_c2c_prime_axes_vs_oracle(gpu, shape, prec):
+    import daggerfft as D
+    dt = np.complex128 if prec else np.complex64
+    tol = TOL64 if prec else TOL32
+    a = O.fill(21, shape, dt)
+    ref_in = a.astype(np.complex128)
+    F = D.fft(_dev(a), decomp=D.Pencil(), dims=(1, 2, 3))
+    e_f = O.rel_l2(F.to_host().astype(np.complex128), O.fftn(ref_in))
+    B = D.ifft(_dev(a), decomp=D.Pencil(), dims=(1, 2, 3))
+    e_b = O.rel_l2(B.to_host().astype(np.complex128), O.fftn(ref_in, inverse=True))
+    R = D.ifft(F, decomp=D.Pencil(), dims=(1, 2, 3))
+    e_r = O.rel_l2(R.to_host().astype(np.complex128), ref_in)
+    assert e_f <= tol and e_b <= tol and e_r <= tol, (e_f, e_b, e_r)
+
+
+@pytest.mark.parametrize("shape", [(11, 13, 17), (30, 97, 8)], ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("dims", [(1, 2, 3), (3, 1, 2), (2,), (1, 3)], ids=lambda d: "".join(map(str, d)))
+def test_c2c_prime_dims_and_slab(gpu, shape, dims):
+    import daggerfft as D
+    a = O.fill(22, shape, np.complex128)
+    ref = O.fftn(a, dims)
+    for dec in (D.Pencil(), D.Slab()):
+        if isinstance(dec, D.Slab) and len(dims) != 3:
+            continue
+        assert O.rel_l2(D.fft(_dev(a), decomp=dec, dims=dims).to_host(), ref) <= TOL64
+
+
+@pytest.mark.parametrize("shape", [(22, 13, 6), (13, 11, 4), (2018, 3, 2), (26, 34, 38), (3, 2, 1)],
+                         ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("prec", [1, 0], ids=["f64", "f32"])
+def test_rfft_irfft_prime_lengths(gpu, shape, prec):
+    """Even n1 with a prime half (packed R2C/C2R around the Bluestein core),
+    odd prime n1 (full-length R2C/C2R), prime edges on the other axes."""
+    import daggerfft as D
+    rdt = np.float64 if prec else np.float32
+    tol = TOL64 if prec else TOL32
+    x = O.fill(23, shape, rdt)
+    F = D.rfft(_dev(x), decomp=D.Pencil(), dims=(1, 2, 3))
+    ref = O.rfftn(x.astype(np.float64))
+    assert O.rel_l2(F.to_host().astype(np.complex128), ref) <= tol
+    y = D.irfft(_dev(ref.astype(np.complex128 if prec else np.complex64)), shape[0], decomp=D.Pencil(),
+                dims=(1, 2, 3))
+    assert O.rel_l2(y.to_host().astype(np.float64), x.astype(np.float64)) <= tol
+
+
+KINDS = ["DHT", "REDFT00", "REDFT01", "REDFT10", "REDFT11", "RODFT00", "RODFT01", "RODFT10", "RODFT11"]
+
+
+@pytest.mark.parametrize("kind", KINDS)
+@pytest.mark.parametrize("shape", [(11, 13), (12, 10, 3), (97, 5)], ids=lambda s: "x".join(map(str, s)))
+def test_r2r_kinds_bluestein(gpu, kind, shape):
+    """Every r2r kind where its FFT length has a prime factor above 7
+    (REDFT00 of n = 12 is a 22-point FFT, RODFT00 of n = 10 too)."""
+    import daggerfft as D
+    a = O.fill(24, shape, np.float64)
+    dims = tuple(range(1, len(shape) + 1))
+    kinds = tuple(getattr(D, kind) for _ in dims)
+    G = D.fft(_dev(a), decomp=D.Pencil(), dims=dims, transforms=D.R2R(kinds))
+    ref = O.r2rn(a, tuple(getattr(O, kind) for _ in dims), dims)
+    assert O.rel_l2(G.to_host(), ref) <= TOL64
+
+
+def test_plan_describes_bluestein_passes(gpu):
+    import daggerfft as D
+    from daggerfft import _lib as L
+    p = D.Plan((11, 16, 1009), L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3))
+    by_axis = {q["axis"]: q for q in p.passes()}
+    assert by_axis[0]["bluestein"] == 21 and by_axis[0]["kernel"].startswith("k_axis_blue<double,0,true")
+    assert by_axis[1]["bluestein"] == 0
+    assert by_axis[2]["bluestein"] >= 2 * 1009 - 1 and by_axis[2]["kernel"].startswith("k_axis_blue<double,0,false")
+
+
+def test_plane_pipeline_and_graph_with_bluestein(gpu):
+    """A Slab plane-pipeline request falls back to one launch per pass (the
+    grouped kernel has no Bluestein tile); graph replay runs the same kernels."""
+    import daggerfft as D
+    from daggerfft import _lib as L
+    shape = (26, 13, 11)
+    a = O.fill(25, shape, np.complex128)
+    ref = O.fftn(a)
+    p = D.Plan(shape, L.C2C, -1, L.F64, D.Slab(), (1, 2, 3), flags=L.FLAG_PLANE_PIPELINE)
+    assert p.schedule()[0] == "passes"
+    out = D.DeviceArray(shape, np.complex128)
+    p.execute(_dev(a), out)
+    assert O.rel_l2(out.to_host(), ref) <= TOL64
+    g = D.Plan(shape, L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3), flags=L.FLAG_GRAPH)
+    X = _dev(a)
+    for _ in range(3):
+        out2 = D.DeviceArray(shape, np.complex128)
+        g.execute(X, out2)
+        assert O.rel_l2(out2.to_host(), ref) <= TOL64
+
+
+def test_batch_with_prime_lengths(gpu):
+    """The default batch schedule runs transforms with a Bluestein axis on
+    per-transform plans (lanes); the grouped schedule refuses them."""
+    import daggerfft as D
+    from daggerfft import _lib as L
+    from daggerfft.batch import Batch
+    descs = [((11, 16, 8), L.C2C, -1, 1), ((32, 13, 17), L.C2C, -1, 0), ((22, 10, 9), L.R2C, -1, 1)]
+    b = Batch(descs)
+    ins, outs, refs = [], [], []
+    for t, (shape, kind, sign, prec) in enumerate(descs):
+        if kind == L.C2C:
+            a = O.fill(30 + t, shape, np.complex128 if prec else np.complex64)
+            outs.append(D.DeviceArray(shape, a.dtype))
+            refs.append(O.fftn(a.astype(np.complex128)))
+        else:
+            a = O.fill(30 + t, shape, np.float64)
+            outs.append(D.DeviceArray((shape[0] // 2 + 1,) + shape[1:], np.complex128))
+            refs.append(O.rfftn(a))
+        ins.append(_dev(a))
+    b.execute(ins, outs)
+    for (shape, kind, sign, prec), Y, ref in zip(descs, outs, refs):
+        assert O.rel_l2(Y.to_host().astype(np.complex128), ref) <= (TOL64 if prec else TOL32)
+    with pytest.raises(D.DFFTError) as ei:
+        Batch(descs[:2], flags=L.FLAG_BATCH_GROUPED)
+    assert ei.value.status == L.DFFT_ERR_UNSUPPORTED_SIZE
+
+
+def test_lengths_beyond_one_tile_are_refused(gpu):
+    """Bluestein needs M >= 2n-1 points of one line in LDS with at most 1024
+    threads: n up to 4096 runs, 4099 (prime, M = 8232) does not."""
+    import daggerfft as D
+    from daggerfft import _lib as L
+    D.Plan((4096 - 3, 2, 1), L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3))
+    with pytest.raises(D.DFFTError) as ei:
+        D.Plan((4099, 2, 1), L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3))
+    assert ei.value.status == L.DFFT_ERR_UNSUPPORTED_SIZE

@@ -113,8 +113,8 @@ def test_request_over_budget_is_rejected(gpu):
         with pytest.raises(L.DFFTError) as e:
             srv.submit(S.Request((128, 128, 128), L.C2C, L.F64, 1))
         assert e.value.status == L.DFFT_ERR_BUDGET
-        with pytest.raises(L.DFFTError) as e:
-            srv.submit(S.Request((22, 16, 16), L.C2C, L.F64, 1))  # prime factor 11
+        with pytest.raises(L.DFFTError) as e:  # too long for one LDS tile
+            srv.submit(S.Request((1, 1, 1 << 15), L.C2C, L.F32, 1))
         assert e.value.status == L.DFFT_ERR_UNSUPPORTED_SIZE
 
 

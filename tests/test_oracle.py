@@ -82,6 +82,20 @@ def test_c2c_vs_scipy(shape):
     assert O.rel_l2(O.fftn(a, inverse=True), sf.ifftn(a)) < 1e-14
 
 
+@pytest.mark.parametrize("shape", [(11, 13, 17), (1009, 3, 2), (2, 3, 4093), (22, 26, 34), (97, 1, 5)])
+def test_c2c_prime_lengths_vs_scipy(shape):
+    """Lengths with prime factors above 7 (the library's Bluestein passes):
+    the oracle's recursive FFT ends in direct DFTs of the prime factors."""
+    a = O.fill(3, shape, np.complex128)
+    assert O.rel_l2(O.fftn(a), sf.fftn(a)) < 1e-14
+    assert O.rel_l2(O.fftn(a, inverse=True), sf.ifftn(a)) < 1e-14
+    r = O.fill(4, shape, np.float64)
+    ref = np.asfortranarray(np.transpose(sf.rfftn(np.transpose(r), axes=(0, 1, 2))))
+    assert O.rel_l2(O.rfftn(r), ref) < 1e-14
+    assert O.rel_l2(O.r2rn(r, O.REDFT10), sf.dctn(r, type=2)) < 1e-14
+    assert O.rel_l2(O.r2rn(r, O.REDFT01), sf.dctn(r, type=3)) < 1e-14
+
+
 def test_rfft_256_cubed_vs_scipy():
     a = O.fill(2, (256, 256, 256), np.float64)
     F = O.rfftn(a)
