@@ -218,17 +218,28 @@ static bool tpl_ok(int L, const std::vector<int> &R, int emax, int tpl) {
 int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_lds, int contig_threads,
                   bool grouped) {
     AxisPass &a = pp.ap;
-    // A length with a prime factor > 7 runs Bluestein's convolution over the
-    // smallest 7-smooth M >= 2n-1 (runtime engine; the grouped kernel has no
-    // Bluestein tile).
+    // A length with a prime factor > 7 runs Bluestein's convolution over a
+    // 7-smooth M >= 2n-1 (runtime engine; the grouped kernel has no Bluestein
+    // tile).  Each of its two FFTs costs about (stages x threads per line)
+    // barrier-separated LDS rounds, so M is the candidate up to twice the
+    // minimum with the least of that product (the smallest M on ties).
     a.blue = 0;
     a.bsp = StagePlan{};
     {
         std::vector<int> f;
         if (!factorize(a.sp.L, f)) {
             if (grouped) return DFFT_ERR_UNSUPPORTED_SIZE;
-            int M = 2 * a.sp.L - 1;
-            while (!factorize(M, f)) ++M;
+            const int m0 = 2 * a.sp.L - 1;
+            int M = 0;
+            int64_t best = 0;
+            for (int m = m0; m <= 2 * m0; ++m) {
+                if (!factorize(m, f)) continue;
+                const int64_t cost = (int64_t)f.size() * default_tpl(m, f, EMAX, EMAX);
+                if (!M || cost < best) {
+                    M = m;
+                    best = cost;
+                }
+            }
             a.blue = 1;
             a.bsp.L = M;
             a.sp.S = 0;
@@ -292,6 +303,20 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
         // small tiles in flight per CU
         const int wg = contig_threads > 256 ? contig_threads : a.variant == 2 ? 64 : 128;
         int tc = (tune && tune->tc > 0) ? tune->tc : std::max(1, wg / tpl);
+        if (a.blue && tpl < 512) {
+            // Bluestein lines have odd thread counts (105 for M = 525): take
+            // the line count in 128..512 threads that leaves the fewest idle
+            // lanes in the last wavefront
+            double util = 0;
+            for (int c = std::max(1, (128 + tpl - 1) / tpl); c * tpl <= 512; ++c) {
+                const int th = c * tpl;
+                const double u = (double)th / (double)(((th + 63) / 64) * 64);
+                if (u > util + 1e-9) {
+                    util = u;
+                    tc = c;
+                }
+            }
+        }
         while (tc > 1 && (int64_t)tc * tpl > max_threads) --tc;
         const int contig_lds = contig_threads > 256 ? lds_limit : std::min(lds_limit, 65536);
         while (tc > 1 && (size_t)tc * a.lds_pitch * lds_es > (size_t)contig_lds) --tc;

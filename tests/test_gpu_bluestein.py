@@ -149,7 +149,8 @@ def test_batch_with_prime_lengths(gpu):
 
 def test_lengths_beyond_one_tile_are_refused(gpu):
     """Bluestein needs M >= 2n-1 points of one line in LDS with at most 1024
-    threads: n up to 4096 runs, 4099 (prime, M = 8232) does not."""
+    threads: n = 4093 runs (M = 8192), 4099 (prime) does not (every 7-smooth
+    M >= 8197 needs more than 1024 threads per line)."""
     import daggerfft as D
     from daggerfft import _lib as L
     D.Plan((4096 - 3, 2, 1), L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3))
