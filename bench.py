@@ -360,6 +360,57 @@ def serving(count):
     out["served_f32_rel_l2_vs_f64_max"] = worst32
     D.clear_plan_cache()
     torch.cuda.empty_cache()
+    out["any_size"] = serving_any_size(count)
+    return out
+
+
+def _smooth(m):
+    for p in (2, 3, 5, 7):
+        while m % p == 0:
+            m //= p
+    return m == 1
+
+
+def serving_any_size(count):
+    """The same stream with the edges rounded to integers instead of snapped
+    to 7-smooth sizes: requests whose edge (or, for R2C, half-edge) has a
+    prime factor above 7 run Bluestein passes.  Saturating throughput against
+    warm sequential plain calls (plans pre-built)."""
+    import torch
+
+    from daggerfft import serve as S
+    from daggerfft import _lib as L
+    import daggerfft as D
+
+    trace = S.serving_trace(count=count, snap=False)
+
+    def blue(q):
+        e = q.shape[0]
+        return not _smooth(e) or (q.kind == L.R2C and e % 2 == 0 and not _smooth(e // 2))
+
+    out = {"trace": "as above with edges rounded to integers (not snapped)",
+           "distinct_shapes": len(set((q.shape, q.kind, q.precision) for q in trace)),
+           "bluestein_requests": sum(blue(q) for q in trace),
+           "algorithmic_bytes_total": float(sum(q.algorithmic_bytes() for q in trace))}
+    with S.Server(hbm_budget=200 * 10 ** 9, max_inflight=32, lanes=4) as srv:
+        S.replay(srv, trace, open_loop=False)  # builds the server's plans
+        sat = S.replay(srv, trace, open_loop=False)
+    sat.pop("results")
+    sat.pop("latencies", None)
+    out["saturating"] = {k: round(v, 4) if isinstance(v, float) else v for k, v in sat.items()
+                         if not k.startswith("latency")}
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    for q in {(q.shape, q.kind, q.precision): q for q in trace}.values():
+        x = D.rand(q.shape, q.in_dtype, seed=1)
+        (D.rfft(x) if q.kind == L.R2C else D.fft(x))
+    torch.cuda.synchronize()
+    warm = S.sequential_plain(trace)
+    warm.pop("results")
+    out["sequential_warm"] = {k: round(v, 4) for k, v in warm.items()}
+    out["throughput_vs_sequential_warm"] = round(sat["req_per_s"] / warm["req_per_s"], 3)
+    D.clear_plan_cache()
+    torch.cuda.empty_cache()
     return out
 
 

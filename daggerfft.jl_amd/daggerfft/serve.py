@@ -90,13 +90,16 @@ def snap_size(x, sizes):
     return a if x - a <= b - x else b
 
 
-def serving_trace(count=2000, seed=8, rate=200.0, lo=64, hi=512, c2c_frac=0.6, f32_frac=0.5, data_seed0=10000):
+def serving_trace(count=2000, seed=8, rate=200.0, lo=64, hi=512, c2c_frac=0.6, f32_frac=0.5, data_seed0=10000,
+                  snap=True):
     """The mixed serving workload.  Request r draws u_a = u(seed, 4r + a) from
     the SplitMix64 counter generator:
       inter-arrival  -ln(1 - u_0) / rate seconds (Poisson arrivals),
       kind           C2C if u_1 < c2c_frac else R2C,
       edge           lo * (hi/lo)^u_2 (log-uniform), snapped to the nearest
-                     even {2,3,5,7}-smooth size; the request is edge^3,
+                     even {2,3,5,7}-smooth size (snap=False: rounded to the
+                     nearest integer, so edges with a prime factor above 7 run
+                     Bluestein passes); the request is edge^3,
       precision      F32 if u_3 < f32_frac else F64,
     and its input is the generator stream of seed data_seed0 + r."""
     from . import _splitmix_u
@@ -107,7 +110,8 @@ def serving_trace(count=2000, seed=8, rate=200.0, lo=64, hi=512, c2c_frac=0.6, f
         u = [_splitmix_u(seed, 4 * r + a) for a in range(4)]
         t += -math.log1p(-u[0]) / rate
         kind = L.C2C if u[1] < c2c_frac else L.R2C
-        edge = snap_size(lo * (hi / lo) ** u[2], sizes)
+        x = lo * (hi / lo) ** u[2]
+        edge = snap_size(x, sizes) if snap else int(round(x))
         prec = L.F32 if u[3] < f32_frac else L.F64
         out.append(Request((edge, edge, edge), kind, prec, data_seed0 + r, t))
     return out

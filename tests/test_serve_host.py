@@ -79,3 +79,18 @@ def test_request_algorithmic_bytes():
     # R2C: real in + half-spectrum out, then two complex passes over the half spectrum
     assert S.Request((64, 64, 64), L.R2C, L.F64, 1).algorithmic_bytes() == n * 8 + nc * 16 + 4 * nc * 16
     assert S.Request((64, 64, 64), L.R2C, L.F32, 1).out_shape == (33, 64, 64)
+
+
+def test_unsnapped_trace_keeps_integer_edges():
+    """snap=False rounds the same log-uniform draws to integers: the arrivals,
+    kinds and precisions are those of the snapped trace, and edges with prime
+    factors above 7 appear (they run Bluestein passes)."""
+    from daggerfft import serve as S
+    a = S.serving_trace(count=300, seed=8)
+    b = S.serving_trace(count=300, seed=8, snap=False)
+    assert [(q.kind, q.precision, q.arrival) for q in a] == [(q.kind, q.precision, q.arrival) for q in b]
+    edges = [q.shape[0] for q in b]
+    assert all(64 <= e <= 512 for e in edges)
+    assert sum(not _smooth(e) for e in edges) > 100
+    for qa, qb in zip(a, b):
+        assert abs(qa.shape[0] - qb.shape[0]) <= max(8, 0.06 * qb.shape[0])
