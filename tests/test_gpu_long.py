@@ -1,0 +1,76 @@
+"""Long lines: the power-of-two lengths 2048..16384 run compile-time
+catalog kernels (16 values per thread, split-LDS exchange, dfft_fixed_long.hip)
+along the contiguous axis, a strided axis with outer slices and a strided
+axis without, in both precisions; FP64 4096 through both of its radix lists.
+ComplexF64 rel-L2 <= 1e-12; ComplexF32 against the FP64 oracle <= 2e-6."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+TOL = {1: 1e-12, 0: 2e-6}
+LONG = [2048, 4096, 8192, 16384]
+
+
+def _case(D, L, shape, prec, ax):
+    dt = np.complex128 if prec else np.complex64
+    a = O.fill(shape[ax] + ax, shape, dt)
+    p = D.Plan(shape, L.C2C, -1, prec, D.Pencil(), (1, 2, 3))
+    q = [x for x in p.passes() if x["axis"] == ax][0]
+    F = p.execute(D.DeviceArray.from_host(a), D.DeviceArray(shape, dt))
+    e = O.rel_l2(F.to_host().astype(np.complex128), O.fftn(a.astype(np.complex128)))
+    pi = D.Plan(shape, L.C2C, +1, prec, D.Pencil(), (1, 2, 3))
+    B = pi.execute(F, F.similar())
+    e2 = O.rel_l2(B.to_host().astype(np.complex128), a.astype(np.complex128))
+    return q, e, e2
+
+
+@pytest.mark.parametrize("n", LONG)
+@pytest.mark.parametrize("prec", [1, 0], ids=["f64", "f32"])
+def test_long_lines_catalog(gpu, n, prec):
+    import daggerfft as D
+    from daggerfft import _lib as L
+    D.tune_clear()
+    for shape, ax in (((n, 3, 2), 0), ((3, n, 2), 1), ((2, 3, n), 2)):
+        print("long", n, "f64" if prec else "f32", shape, flush=True)
+        q, e, e2 = _case(D, L, shape, prec, ax)
+        assert q["variant"] == 2 and q["L"] == n, q
+        assert e <= TOL[prec] and e2 <= TOL[prec], (shape, e, e2)
+    if prec == 1 and n == 4096:
+        assert q["radices"] == "8x8x8x8", q  # the default list of FP64 4096 (16x16x16 spills)
+
+
+def test_f64_4096_both_radix_lists(gpu, tmp_path):
+    """FP64 4096 holds 8x8x8x8 (default) and 16x16x16 (forced through the
+    schedule cache, key prec/op/L/layout/size class)."""
+    import daggerfft as D
+    from daggerfft import _lib as L
+    path = os.path.join(str(tmp_path), "tune.txt")
+    try:
+        for shape, ax, layout in (((4096, 3, 2), 0, 1), ((3, 4096, 2), 1, 2), ((2, 3, 4096), 2, 0)):
+            sc = int(math.floor(math.log2(6 * 4096)))
+            with open(path, "w") as f:
+                f.write("1/0/4096/%d/%d 16,16,16 0 0 0 2 1\n" % (layout, sc))
+            D.tune_clear()
+            D.tune_load(path)
+            q, e, e2 = _case(D, L, shape, 1, ax)
+            assert q["radices"] == "16x16x16" and q["variant"] == 2, q
+            assert e <= 1e-12 and e2 <= 1e-12, (shape, e, e2)
+    finally:
+        D.tune_clear()
+
+
+def test_longest_lines(gpu):
+    """16384 points is the longest line (one LDS tile, 1024 threads); a longer
+    power of two is refused with DFFT_ERR_UNSUPPORTED_SIZE."""
+    import daggerfft as D
+    from daggerfft import _lib as L
+    for prec in (1, 0):
+        D.Plan((16384, 1, 1), L.C2C, -1, prec, D.Pencil(), (1, 2, 3))
+        with pytest.raises(D.DFFTError) as ei:
+            D.Plan((32768, 1, 1), L.C2C, -1, prec, D.Pencil(), (1, 2, 3))
+        assert ei.value.status == L.DFFT_ERR_UNSUPPORTED_SIZE
