@@ -82,6 +82,7 @@ struct TileCtx {
     __device__ __forceinline__ int len() const { return a.sp.L; }
     __device__ __forceinline__ int tpl() const { return a.TPL; }
     __device__ __forceinline__ const cx<T> *twb() const { return reinterpret_cast<const cx<T> *>(a.tw); }
+    __device__ __forceinline__ int stage_tw(int s) const { return a.sp.tw[s]; }
 
     // value entering the FFT at position p (stage 0)
     __device__ __forceinline__ cx<T> load0(int p) const {
@@ -294,8 +295,8 @@ __device__ __forceinline__ void dispatch_stage(const C &c, int R, int Ns, int tw
 // round through the same buffer.
 enum LdsPart : int { PART_FULL = 0, PART_RE = 1, PART_IM = 2 };
 
-template <int R, int EMAX, int LC, int TPLC, class T, int OP, bool CONTIG>
-__device__ __forceinline__ void st_load_global(const TileCtx<T, OP, CONTIG> &c, cx<T> *v) {
+template <int R, int EMAX, int LC, int TPLC, class C>
+__device__ __forceinline__ void st_load_global(const C &c, cx<typename C::value_type> *v) {
     constexpr int NBM = EMAX / R;
     const int NB = LC / R, TPL = TPLC;
 #pragma unroll
@@ -309,8 +310,9 @@ __device__ __forceinline__ void st_load_global(const TileCtx<T, OP, CONTIG> &c, 
 }
 
 // twiddle by w_{Ns*R}^{r (j mod Ns)} (not in the first stage), then the R-point DFT
-template <int R, int EMAX, int LC, int TPLC, int Ns, class T, int OP, bool CONTIG>
-__device__ __forceinline__ void st_compute(const TileCtx<T, OP, CONTIG> &c, cx<T> *v, int twoff) {
+template <int R, int EMAX, int LC, int TPLC, int Ns, class C>
+__device__ __forceinline__ void st_compute(const C &c, cx<typename C::value_type> *v, int twoff) {
+    typedef typename C::value_type T;
     constexpr bool twiddle = Ns > 1;
     constexpr int NBM = EMAX / R;
     const int NB = LC / R, TPL = TPLC;
@@ -320,7 +322,7 @@ __device__ __forceinline__ void st_compute(const TileCtx<T, OP, CONTIG> &c, cx<T
         if (j < NB) {
             if constexpr (twiddle) {
                 const int k = j % Ns;
-                const cx<T> *w = reinterpret_cast<const cx<T> *>(c.a.tw) + twoff + k * (R - 1);
+                const cx<T> *w = c.twb() + twoff + k * (R - 1);
 #pragma unroll
                 for (int r = 1; r < R; ++r) v[b * R + r] = cmul(v[b * R + r], w[r - 1]);
             }
@@ -329,8 +331,8 @@ __device__ __forceinline__ void st_compute(const TileCtx<T, OP, CONTIG> &c, cx<T
     }
 }
 
-template <int R, int EMAX, int LC, int TPLC, class T, int OP, bool CONTIG>
-__device__ __forceinline__ void st_store_global(const TileCtx<T, OP, CONTIG> &c, const cx<T> *v) {
+template <int R, int EMAX, int LC, int TPLC, class C>
+__device__ __forceinline__ void st_store_global(const C &c, const cx<typename C::value_type> *v) {
     constexpr int NBM = EMAX / R;
     const int NB = LC / R, TPL = TPLC;
 #pragma unroll
@@ -344,8 +346,9 @@ __device__ __forceinline__ void st_store_global(const TileCtx<T, OP, CONTIG> &c,
 }
 
 // Stockham write of stage outputs: y[(j - j mod Ns) R + j mod Ns + r Ns]
-template <int R, int EMAX, int PART, int LC, int TPLC, int Ns, class T, int OP, bool CONTIG>
-__device__ __forceinline__ void st_write_lds(const TileCtx<T, OP, CONTIG> &c, const cx<T> *v) {
+template <int R, int EMAX, int PART, int LC, int TPLC, int Ns, class C>
+__device__ __forceinline__ void st_write_lds(const C &c, const cx<typename C::value_type> *v) {
+    typedef typename C::value_type T;
     constexpr int NBM = EMAX / R;
     const int NB = LC / R, TPL = TPLC;
     T *ldsr = reinterpret_cast<T *>(c.lds);
@@ -367,8 +370,9 @@ __device__ __forceinline__ void st_write_lds(const TileCtx<T, OP, CONTIG> &c, co
 }
 
 // Stockham read of stage inputs: x[j + r L/R]
-template <int R, int EMAX, int PART, int LC, int TPLC, class T, int OP, bool CONTIG>
-__device__ __forceinline__ void st_read_lds(const TileCtx<T, OP, CONTIG> &c, cx<T> *v) {
+template <int R, int EMAX, int PART, int LC, int TPLC, class C>
+__device__ __forceinline__ void st_read_lds(const C &c, cx<typename C::value_type> *v) {
+    typedef typename C::value_type T;
     constexpr int NBM = EMAX / R;
     const int NB = LC / R, TPL = TPLC;
     const T *ldsr = reinterpret_cast<const T *>(c.lds);
@@ -481,8 +485,8 @@ constexpr int fixed_tpl() {
     return t;
 }
 
-template <class RL, int EMAX, int TPLC, bool SPLIT, int s, class T, int OP, bool CONTIG>
-__device__ __forceinline__ void fixed_stages_from(const TileCtx<T, OP, CONTIG> &c, cx<T> *v) {
+template <class RL, int EMAX, int TPLC, bool SPLIT, int s, class C>
+__device__ __forceinline__ void fixed_stages_from(const C &c, cx<typename C::value_type> *v) {
     if constexpr (s + 1 < RL::S) {
         constexpr int Rs = RL::R[s], Rn = RL::R[s + 1], Ns = RL::ns(s), L = RL::L;
         if constexpr (!SPLIT) {
@@ -500,7 +504,7 @@ __device__ __forceinline__ void fixed_stages_from(const TileCtx<T, OP, CONTIG> &
             st_read_lds<Rn, EMAX, PART_IM, L, TPLC>(c, v);
             __syncthreads();
         }
-        st_compute<Rn, EMAX, L, TPLC, Ns * Rs>(c, v, c.a.sp.tw[s + 1]);
+        st_compute<Rn, EMAX, L, TPLC, Ns * Rs>(c, v, c.stage_tw(s + 1));
         fixed_stages_from<RL, EMAX, TPLC, SPLIT, s + 1>(c, v);
     }
 }
@@ -703,6 +707,11 @@ struct BlueCtx : TileCtx<T, OP, CONTIG> {
     __device__ __forceinline__ BlueCtx(const Op &c)
         : Op(c), chirp(c.twb() + c.a.chirp), bhat(c.twb() + c.a.bhat), n(c.a.sp.L), M(c.a.bsp.L) {}
     __device__ __forceinline__ int len() const { return M; }
+    // stage twiddle blocks of the compile-time engine: the first FFT's
+    // schedule, then its reverse for the second
+    __device__ __forceinline__ int stage_tw(int s) const {
+        return PHASE == 0 ? this->a.bsp.tw[s] : this->a.bsp2.tw[s];
+    }
 
     __device__ __forceinline__ cx<T> load0(int p) const {  // phase 0 only
         if (p >= n) return cx<T>{0, 0};
@@ -742,6 +751,59 @@ __device__ __forceinline__ void blue_tile(const AxisPass &a, const void *in, voi
     __syncthreads();
     blue_fft<EMAX, true>(BlueCtx<T, OP, CONTIG, 1>(c), a.bsp);
     if constexpr (OP == OP_R2C) r2c_post_lds(c, a.TPL);
+}
+
+// A radix list reversed (a valid Stockham schedule of the same length).
+template <class RL, class Seq>
+struct ReversedImpl;
+template <class RL, int... I>
+struct ReversedImpl<RL, std::integer_sequence<int, I...>> {
+    using type = Radices<RL::R[RL::S - 1 - I]...>;
+};
+template <class RL>
+using Reversed = typename ReversedImpl<RL, std::make_integer_sequence<int, RL::S>>::type;
+
+// Between the two FFTs of a compile-time Bluestein tile.  The first FFT's
+// last stage (radix R) leaves output p = j + r*M/R of butterfly j in
+// v[b*R + r]; the second FFT runs the reversed radix list, so its first
+// stage has the same radix and reads exactly those positions into the same
+// registers: conj(A_p * bhat_p) is formed in place, with no LDS round trip.
+template <int R, int EMAX, int LC, int TPLC, class C>
+__device__ __forceinline__ void blue_turn(const C &c, cx<typename C::value_type> *v) {
+    constexpr int NBM = EMAX / R;
+    const int NB = LC / R;
+#pragma unroll
+    for (int b = 0; b < NBM; ++b) {
+        const int j = c.t + b * TPLC;
+        if (j < NB) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b * R + r] = conj(cmul(v[b * R + r], c.bhat[j + r * NB]));
+        }
+    }
+}
+
+// Bluestein tile of a C2C pass on the compile-time engine: convolution
+// length RL::L (a power of two), 16 values per thread, split LDS exchange.
+template <class RL, class T, bool CONTIG>
+__device__ __forceinline__ void blue_fixed_tile(const AxisPass &a, const void *in, void *out, int64_t tile, int tid,
+                                                cx<T> *lds) {
+    constexpr int EMAX = 16, TPLC = fixed_tpl<RL, EMAX>(), M = RL::L;
+    using RR = Reversed<RL>;
+    int ls, t;
+    bool valid;
+    int64_t ib, ob;
+    tile_map<CONTIG>(a, tile, tid, TPLC, ls, t, valid, ib, ob);
+    TileCtx<T, OP_C2C, CONTIG> c{a, in, out, lds, ls, t, valid, ib, ob};
+    const BlueCtx<T, OP_C2C, CONTIG, 0> c0(c);
+    const BlueCtx<T, OP_C2C, CONTIG, 1> c1(c);
+    cx<T> v[EMAX];
+    st_load_global<RL::R[0], EMAX, M, TPLC>(c0, v);
+    st_compute<RL::R[0], EMAX, M, TPLC, 1>(c0, v, 0);
+    fixed_stages_from<RL, EMAX, TPLC, true, 0>(c0, v);
+    blue_turn<RL::R[RL::S - 1], EMAX, M, TPLC>(c0, v);
+    st_compute<RR::R[0], EMAX, M, TPLC, 1>(c1, v, 0);
+    fixed_stages_from<RR, EMAX, TPLC, true, 0>(c1, v);
+    st_store_global<RR::R[RR::S - 1], EMAX, M, TPLC>(c1, v);
 }
 
 }  // namespace dfft

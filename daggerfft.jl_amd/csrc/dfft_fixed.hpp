@@ -41,8 +41,10 @@ struct FixedTable {
     int n;
 };
 
-// the Bluestein kernel of (precision, layout, op), for AxisPass::blue passes (dfft_blue.hip)
+// the Bluestein kernels (dfft_blue.hip): runtime engine of (precision, layout,
+// op); compile-time C2C engine of (precision, layout, convolution length)
 axis_kernel_t blue_kernel(int prec, int contig, int op);
+axis_kernel_t blue_fixed_kernel(int prec, int contig, int M);
 
 #define DFFT_FX(T, P, OPV, CONTIG, VAR, EMAXV, SPLIT, RL)                                             \
     {P, OPV, CONTIG, VAR, RL::L, fixed_tpl<RL, EMAXV>(), RL::S, RL::R,                                 \

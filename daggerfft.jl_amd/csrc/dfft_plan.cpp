@@ -120,6 +120,7 @@ static void build_twiddles(StagePlan &sp, int op, int nlog, std::vector<long dou
 static void build_bluestein(AxisPass &ap, std::vector<long double> &tab) {
     const int n = ap.sp.L, M = ap.bsp.L;
     build_twiddles(ap.bsp, OP_C2C, M, tab);
+    if (ap.variant) build_twiddles(ap.bsp2, OP_C2C, M, tab);
     ap.chirp = (int)(tab.size() / 2);
     std::vector<long double> br(n), bi(n);
     for (int p = 0; p < n; ++p) {
@@ -156,19 +157,20 @@ void TwiddleTable::add(AxisPass &ap) {
     std::string key = std::to_string(ap.op) + ":" + std::to_string(ap.sp.L);
     for (int s = 0; s < ap.sp.S; ++s) key += "," + std::to_string(ap.sp.R[s]);
     if (ap.blue) {
-        key += "/B" + std::to_string(ap.bsp.L);
+        key += "/B" + std::to_string(ap.variant) + ":" + std::to_string(ap.bsp.L);
         for (int s = 0; s < ap.bsp.S; ++s) key += "," + std::to_string(ap.bsp.R[s]);
     }
     auto it = seen.find(key);
     if (it != seen.end()) {
         ap.sp = it->second.sp;
         ap.bsp = it->second.bsp;
+        ap.bsp2 = it->second.bsp2;
         ap.chirp = it->second.chirp;
         ap.bhat = it->second.bhat;
     } else {
         build_twiddles(ap.sp, ap.op, ap.nlog, tab);
         if (ap.blue) build_bluestein(ap, tab);
-        seen[key] = TwiddleBlocks{ap.sp, ap.bsp, ap.chirp, ap.bhat};
+        seen[key] = TwiddleBlocks{ap.sp, ap.bsp, ap.bsp2, ap.chirp, ap.bhat};
     }
 }
 
@@ -223,25 +225,53 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
     // tile).  Each of its two FFTs costs about (stages x threads per line)
     // barrier-separated LDS rounds, so M is the candidate up to twice the
     // minimum with the least of that product (the smallest M on ties).
+    // A C2C pass takes the compile-time engine instead (both FFTs unrolled,
+    // the exchange between them in registers) over the catalog length
+    // (2^k, 3*2^k, 5*2^k) >= 2n-1 with the least stages x threads per line,
+    // unless the schedule cache names the runtime engine (variant 0).
     a.blue = 0;
     a.bsp = StagePlan{};
+    a.bsp2 = StagePlan{};
+    std::vector<int> blue_fixed_R;
+    int blue_fixed_tpl = 0;
     {
         std::vector<int> f;
         if (!factorize(a.sp.L, f)) {
             if (grouped) return DFFT_ERR_UNSUPPORTED_SIZE;
             const int m0 = 2 * a.sp.L - 1;
-            int M = 0;
-            int64_t best = 0;
-            for (int m = m0; m <= 2 * m0; ++m) {
-                if (!factorize(m, f)) continue;
-                const int64_t cost = (int64_t)f.size() * default_tpl(m, f, EMAX, EMAX);
-                if (!M || cost < best) {
-                    M = m;
-                    best = cost;
+            const bool runtime = tune && tune->variant == 0;
+            a.blue = 1;
+            int Mf = 0;
+            if (a.op == OP_C2C && !runtime) {
+                int64_t best = 0;
+                for (int m : blue_fixed_lengths(pp.prec, a.contig)) {
+                    std::vector<int> r;
+                    int t = 0;
+                    if (m < m0 || !blue_fixed_schedule(pp.prec, a.contig, m, r, t)) continue;
+                    const int64_t cost = (int64_t)r.size() * t;
+                    if (!Mf || cost < best || (cost == best && m < Mf)) {
+                        Mf = m;
+                        best = cost;
+                    }
                 }
             }
-            a.blue = 1;
-            a.bsp.L = M;
+            if (Mf) {
+                (void)blue_fixed_schedule(pp.prec, a.contig, Mf, blue_fixed_R, blue_fixed_tpl);
+                a.bsp.L = Mf;
+            } else {
+                blue_fixed_R.clear();
+                int M = 0;
+                int64_t best = 0;
+                for (int m = m0; m <= 2 * m0; ++m) {
+                    if (!factorize(m, f)) continue;
+                    const int64_t cost = (int64_t)f.size() * default_tpl(m, f, EMAX, EMAX);
+                    if (!M || cost < best) {
+                        M = m;
+                        best = cost;
+                    }
+                }
+                a.bsp.L = M;
+            }
             a.sp.S = 0;
             tune = nullptr;
         }
@@ -274,6 +304,11 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
             break;
         }
     }
+    if (!blue_fixed_R.empty()) {
+        a.variant = 2;
+        R = blue_fixed_R;
+        fixed_tpl = blue_fixed_tpl;
+    }
     if (!a.variant) {
         // a tuned radix list for the runtime engine (radices up to 8); a
         // catalog kernel's list (variant >= 1) may hold composite radices
@@ -285,6 +320,10 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
     if (prod != L || (int)R.size() > MAX_STAGES) return DFFT_ERR_UNSUPPORTED_SIZE;
     fp.S = (int)R.size();
     for (int s = 0; s < fp.S; ++s) fp.R[s] = R[s];
+    if (a.blue && a.variant) {
+        a.bsp2 = a.bsp;
+        std::reverse(a.bsp2.R, a.bsp2.R + a.bsp2.S);
+    }
     const int emax = a.variant >= 2 ? 16 : EMAX;
     const int lds_es = a.variant == 2 ? es / 2 : es;
     int tpl = a.variant ? fixed_tpl
@@ -898,7 +937,9 @@ int dfft_plan_describe(const dfft_plan *p, int pass, char *buf, size_t buflen) {
     // the kernel's name as profilers print it (spaces removed)
     const char *T = pp.prec ? "double" : "float";
     std::string kname;
-    if (a.blue)
+    if (a.blue && a.variant)
+        kname = std::string("k_blue_fixed<") + T + "," + (a.contig ? "true" : "false") + ",dfft::Radices<" + rl + ">>";
+    else if (a.blue)
         kname = std::string("k_axis_blue<") + T + "," + std::to_string(a.op) + "," + (a.contig ? "true" : "false") + ">";
     else if (a.variant >= 1)
         kname = std::string("k_axis_fixed<") + T + "," + std::to_string(a.op) + "," + (a.contig ? "true" : "false") +

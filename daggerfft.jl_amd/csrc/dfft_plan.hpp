@@ -50,7 +50,7 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads = 1024, size
 
 // Offsets of one pass's blocks in a TwiddleTable.
 struct TwiddleBlocks {
-    StagePlan sp, bsp;
+    StagePlan sp, bsp, bsp2;
     int chirp = 0, bhat = 0;
 };
 

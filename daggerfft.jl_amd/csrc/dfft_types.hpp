@@ -74,6 +74,9 @@ struct AxisPass {
     int chirp;                // offset of c_p = exp(-i pi p^2 / sp.L), p < sp.L
     int bhat;                 // offset of FFT_M(b)_k / M, b_m = conj(c_m) = b_{M-m}, M = bsp.L
     StagePlan bsp;            // schedule of the convolution length M
+    StagePlan bsp2;           // compile-time Bluestein (variant 2): the second FFT's schedule,
+                              // bsp's radices reversed (its first stage takes the first FFT's
+                              // last-stage outputs straight from registers)
 };
 
 }  // namespace dfft

@@ -1,7 +1,9 @@
 """Bluestein (chirp-z) passes: line lengths with a prime factor above 7,
 which FFTW and cuFFT accept and the Stockham catalog cannot factor.  Each
-such axis runs as two M-point FFTs of a cyclic convolution in LDS (M >= 2n-1,
-7-smooth), around the same fused load/store maps as every other pass.
+such axis runs as two M-point FFTs of a cyclic convolution in LDS (M >= 2n-1),
+around the same fused load/store maps as every other pass: C2C passes on the
+compile-time engine (M one of 2^k, 3*2^k, 5*2^k), the others -- and C2C when the
+schedule cache names it -- on the runtime engine (M 7-smooth).
 
 Parity with the oracle (whose recursive FFT ends in direct DFTs of the prime
 factors, pinned against scipy in test_oracle.py): ComplexF64/Float64 rel-L2
@@ -89,14 +91,76 @@ def test_r2r_kinds_bluestein(gpu, kind, shape):
     assert O.rel_l2(G.to_host(), ref) <= TOL64
 
 
+MIXED_M_EDGES = [257, 513, 769, 1025, 1537, 2049]
+
+
+@pytest.mark.parametrize("n", MIXED_M_EDGES)
+@pytest.mark.parametrize("prec", [1, 0], ids=["f64", "f32"])
+def test_c2c_compile_time_convolution_lengths(gpu, n, prec):
+    """Edges whose cheapest compile-time convolution length is a power of two
+    or 3*2^k / 5*2^k (257 -> 640, 513 -> 1280, 1025 -> 2560, ...), along
+    each layout, against the oracle."""
+    import daggerfft as D
+    from daggerfft import _lib as L
+    dt = np.complex128 if prec else np.complex64
+    tol = TOL64 if prec else TOL32
+    for shape, ax in (((n, 3, 2), 0), ((3, n, 2), 1), ((2, 3, n), 2)):
+        a = O.fill(n + ax, shape, dt)
+        p = D.Plan(shape, L.C2C, -1, prec, D.Pencil(), (1, 2, 3))
+        q = [x for x in p.passes() if x["axis"] == ax][0]
+        assert q["kernel"].startswith("k_blue_fixed") and q["bluestein"] >= 2 * n - 1, q
+        F = p.execute(_dev(a), D.DeviceArray(shape, dt))
+        e = O.rel_l2(F.to_host().astype(np.complex128), O.fftn(a.astype(np.complex128)))
+        assert e <= tol, (shape, q["bluestein"], e)
+
+
 def test_plan_describes_bluestein_passes(gpu):
+    """C2C passes run the compile-time engine over the next power of two;
+    R2C's first pass runs the runtime engine over a 7-smooth length."""
     import daggerfft as D
     from daggerfft import _lib as L
     p = D.Plan((11, 16, 1009), L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3))
     by_axis = {q["axis"]: q for q in p.passes()}
-    assert by_axis[0]["bluestein"] == 21 and by_axis[0]["kernel"].startswith("k_axis_blue<double,0,true")
+    assert by_axis[0]["bluestein"] == 32 and by_axis[0]["kernel"].startswith("k_blue_fixed<double,true")
+    assert D.Plan((257, 2, 2), L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3)).passes()[0]["bluestein"] == 640
     assert by_axis[1]["bluestein"] == 0
-    assert by_axis[2]["bluestein"] >= 2 * 1009 - 1 and by_axis[2]["kernel"].startswith("k_axis_blue<double,0,false")
+    assert by_axis[2]["bluestein"] == 2048 and by_axis[2]["kernel"].startswith("k_blue_fixed<double,false")
+    r = D.Plan((22, 16, 8), L.R2C, -1, L.F64, D.Pencil(), (1, 2, 3))
+    q = r.passes()[0]
+    assert q["op"] == "R2C" and q["bluestein"] == 21 and q["kernel"].startswith("k_axis_blue<double,1,true")
+
+
+@pytest.mark.parametrize("shape", [(11, 13, 17), (97, 6, 5), (4, 3, 1031), (2003, 2, 2)], ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("prec", [1, 0], ids=["f64", "f32"])
+def test_c2c_runtime_bluestein_forced(gpu, tmp_path, shape, prec):
+    """The runtime-engine Bluestein C2C pass, named through the schedule
+    cache (variant 0) for every axis, against the oracle."""
+    import math
+    import os
+    import daggerfft as D
+    from daggerfft import _lib as L
+    lines = []
+    for ax in range(3):
+        n = shape[ax]
+        layout = 1 if ax == 0 else (2 if ax == 1 and shape[2] > 1 else 0)
+        sc = int(math.floor(math.log2(max(1, np.prod(shape)))))
+        lines.append("%d/0/%d/%d/%d - 0 0 0 0 1" % (prec, n, layout, sc))
+    path = os.path.join(str(tmp_path), "tune.txt")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    dt = np.complex128 if prec else np.complex64
+    tol = TOL64 if prec else TOL32
+    try:
+        D.tune_clear()
+        D.tune_load(path)
+        p = D.Plan(shape, L.C2C, -1, prec, D.Pencil(), (1, 2, 3))
+        kinds = [q["kernel"].split("<")[0] for q in p.passes() if q["bluestein"]]
+        assert kinds and all(k == "k_axis_blue" for k in kinds), p.passes()
+        a = O.fill(26, shape, dt)
+        F = p.execute(_dev(a), D.DeviceArray(shape, dt))
+        assert O.rel_l2(F.to_host().astype(np.complex128), O.fftn(a.astype(np.complex128))) <= tol
+    finally:
+        D.tune_clear()
 
 
 def test_plane_pipeline_and_graph_with_bluestein(gpu):
