@@ -269,6 +269,40 @@ def configs():
     out["batch64_mixed_c2c"] = {"transforms": 64, "default_schedule": "grouped" if Batch(descs).launches == 3 else "lanes",
                                 **res, "sequential_plans_ms": round(seq, 4),
                                 "f32_rel_l2_vs_f64_max": worst32, "algorithmic_bytes": b.algorithmic_bytes()}
+    del ins, outs
+    torch.cuda.empty_cache()
+    out["bluestein"] = _forward_table([((257,) * 3, 1), ((257,) * 3, 0), ((251,) * 3, 1), ((127,) * 3, 1),
+                                       ((131,) * 3, 1), ((1009, 128, 128), 1), ((512, 512, 509), 1)])
+    out["long_lines"] = _forward_table([((4096, 256, 256), 1), ((256, 4096, 128), 1), ((128, 128, 4096), 1),
+                                        ((16384, 64, 64), 1), ((2048, 512, 512), 1)])
+    return out
+
+
+def _forward_table(cases):
+    """3D C2C forward (Pencil) of each (shape, precision): ms, algorithmic
+    GB/s, per-pass convolution length (Bluestein passes) and kernel, and the
+    round-trip error."""
+    import numpy as np
+    import torch
+
+    import daggerfft as D
+    from daggerfft import _lib as L
+    out = {}
+    for sh, prec in cases:
+        dt = np.complex128 if prec else np.complex64
+        A = D.rand(sh, dt, seed=5)
+        X, Y = A.similar(), A.similar()
+        pf = D.Plan(sh, L.C2C, -1, prec, D.Pencil(), (1, 2, 3))
+        pi = D.Plan(sh, L.C2C, +1, prec, D.Pencil(), (1, 2, 3))
+        pi.execute(pf.execute(A, X), Y)
+        torch.cuda.synchronize()
+        rt = float(torch.linalg.vector_norm(Y.tensor - A.tensor) / torch.linalg.vector_norm(A.tensor))
+        ms = _time_events(lambda: pf.execute(A, X), 10)
+        out["%s_%s" % ("x".join(map(str, sh)), "f64" if prec else "f32")] = {
+            "ms": round(ms, 4), "GBps": round(pf.algorithmic_bytes() / ms / 1e6, 1), "roundtrip_rel_l2": rt,
+            "passes": [{"axis": q["axis"], "L": q["L"], "M": q["bluestein"], "kernel": q["kernel"].split("<")[0]}
+                       for q in pf.passes()]}
+        del A, X, Y
     torch.cuda.empty_cache()
     return out
 
