@@ -141,7 +141,12 @@ int dfft_plan_destroy(dfft_plan *plan);
 int dfft_execute_pass(const dfft_plan *plan, int pass, const void *d_in, void *d_out, void *d_work,
                       void *stream);
 
-/* Number of axis passes and their names/geometry, for profiling and tests. */
+/* Number of axis passes and their names/geometry, for profiling and tests.
+ * dfft_plan_describe writes one JSON object per pass: axis, op, L (line
+ * length), radices, TPL/TC (threads per line, lines or columns per tile),
+ * variant (0 runtime schedule, >= 1 compile-time), launch shape, LDS bytes,
+ * algorithmic bytes, bluestein (the convolution length of a Bluestein pass,
+ * 0 otherwise) and the kernel's name as profilers print it. */
 int dfft_plan_num_passes(const dfft_plan *plan, int *npasses);
 int dfft_plan_describe(const dfft_plan *plan, int pass, char *buf, size_t buflen);
 
