@@ -782,28 +782,33 @@ __device__ __forceinline__ void blue_turn(const C &c, cx<typename C::value_type>
     }
 }
 
-// Bluestein tile of a C2C pass on the compile-time engine: convolution
-// length RL::L (a power of two), 16 values per thread, split LDS exchange.
-template <class RL, class T, bool CONTIG>
+// Bluestein tile on the compile-time engine: convolution length RL::L,
+// 16 values per thread, around the op's load/store maps.  SPLIT: the
+// exchanges move real and imaginary parts in two rounds through half the
+// LDS; the packed R2C op needs whole complex values in LDS for its
+// post-twiddle, so it runs the full exchange.
+template <class RL, class T, int OP, bool CONTIG, bool SPLIT>
 __device__ __forceinline__ void blue_fixed_tile(const AxisPass &a, const void *in, void *out, int64_t tile, int tid,
                                                 cx<T> *lds) {
+    static_assert(!(OP == OP_R2C && SPLIT), "the packed R2C post-twiddle reads complex LDS slots");
     constexpr int EMAX = 16, TPLC = fixed_tpl<RL, EMAX>(), M = RL::L;
     using RR = Reversed<RL>;
     int ls, t;
     bool valid;
     int64_t ib, ob;
     tile_map<CONTIG>(a, tile, tid, TPLC, ls, t, valid, ib, ob);
-    TileCtx<T, OP_C2C, CONTIG> c{a, in, out, lds, ls, t, valid, ib, ob};
-    const BlueCtx<T, OP_C2C, CONTIG, 0> c0(c);
-    const BlueCtx<T, OP_C2C, CONTIG, 1> c1(c);
+    TileCtx<T, OP, CONTIG> c{a, in, out, lds, ls, t, valid, ib, ob};
+    const BlueCtx<T, OP, CONTIG, 0> c0(c);
+    const BlueCtx<T, OP, CONTIG, 1> c1(c);
     cx<T> v[EMAX];
     st_load_global<RL::R[0], EMAX, M, TPLC>(c0, v);
     st_compute<RL::R[0], EMAX, M, TPLC, 1>(c0, v, 0);
-    fixed_stages_from<RL, EMAX, TPLC, true, 0>(c0, v);
+    fixed_stages_from<RL, EMAX, TPLC, SPLIT, 0>(c0, v);
     blue_turn<RL::R[RL::S - 1], EMAX, M, TPLC>(c0, v);
     st_compute<RR::R[0], EMAX, M, TPLC, 1>(c1, v, 0);
-    fixed_stages_from<RR, EMAX, TPLC, true, 0>(c1, v);
+    fixed_stages_from<RR, EMAX, TPLC, SPLIT, 0>(c1, v);
     st_store_global<RR::R[RR::S - 1], EMAX, M, TPLC>(c1, v);
+    if constexpr (OP == OP_R2C) r2c_post_lds(c, TPLC);
 }
 
 }  // namespace dfft

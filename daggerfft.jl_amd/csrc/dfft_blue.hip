@@ -42,40 +42,42 @@ axis_kernel_t blue_kernel(int prec, int contig, int op) {
     return g_blue_kernels[prec ? 1 : 0][contig ? 1 : 0][op];
 }
 
-// C2C Bluestein on the compile-time engine (k_blue_fixed, dfft_blue_fixed.hpp):
-// convolution lengths 2^k (32..8192) here, 3*2^k and 5*2^k in
-// dfft_blue_fixed2.hip
+// Bluestein on the compile-time engine (k_blue_fixed, dfft_blue_fixed.hpp):
+// C2C of convolution lengths 2^k (32..8192) here, 3*2^k and 5*2^k in
+// dfft_blue_fixed2.hip, the real ops in dfft_blue_real64/32.hip
 DFFT_TABLE(g_blue_pow2, DFFT_BLUE_M(32), DFFT_BLUE_M(64), DFFT_BLUE_M(128), DFFT_BLUE_M(256), DFFT_BLUE_M(512),
            DFFT_BLUE_M(1024), DFFT_BLUE_M(2048), DFFT_BLUE_M(4096), DFFT_BLUE_M(8192))
-extern const FixedTable g_blue_mixed;
+extern const FixedTable g_blue_mixed, g_blue_real64, g_blue_real32;
 
-static const FixedEntry *blue_fixed_entry(int prec, int contig, int M) {
-    for (const FixedTable *t : {&g_blue_pow2, &g_blue_mixed})
+static const FixedEntry *blue_fixed_entry(int prec, int contig, int op, int M) {
+    for (const FixedTable *t : {&g_blue_pow2, &g_blue_mixed, &g_blue_real64, &g_blue_real32})
         for (int i = 0; i < t->n; ++i) {
             const FixedEntry &e = t->e[i];
-            if (e.prec == (prec ? 1 : 0) && e.contig == (contig ? 1 : 0) && e.L == M) return &e;
+            if (e.prec == (prec ? 1 : 0) && e.contig == (contig ? 1 : 0) && e.op == op && e.L == M) return &e;
         }
     return nullptr;
 }
 
-std::vector<int> blue_fixed_lengths(int prec, int contig) {
+std::vector<int> blue_fixed_lengths(int prec, int contig, int op) {
     std::vector<int> out;
-    for (const FixedTable *t : {&g_blue_pow2, &g_blue_mixed})
+    for (const FixedTable *t : {&g_blue_pow2, &g_blue_mixed, &g_blue_real64, &g_blue_real32})
         for (int i = 0; i < t->n; ++i)
-            if (t->e[i].prec == (prec ? 1 : 0) && t->e[i].contig == (contig ? 1 : 0)) out.push_back(t->e[i].L);
+            if (t->e[i].prec == (prec ? 1 : 0) && t->e[i].contig == (contig ? 1 : 0) && t->e[i].op == op)
+                out.push_back(t->e[i].L);
     return out;
 }
 
-bool blue_fixed_schedule(int prec, int contig, int M, std::vector<int> &radices, int &tpl) {
-    const FixedEntry *e = blue_fixed_entry(prec, contig, M);
+bool blue_fixed_schedule(int prec, int contig, int op, int M, std::vector<int> &radices, int &tpl, int &variant) {
+    const FixedEntry *e = blue_fixed_entry(prec, contig, op, M);
     if (!e) return false;
     radices.assign(e->R, e->R + e->S);
     tpl = e->tpl;
+    variant = e->variant;
     return true;
 }
 
-axis_kernel_t blue_fixed_kernel(int prec, int contig, int M) {
-    const FixedEntry *e = blue_fixed_entry(prec, contig, M);
+axis_kernel_t blue_fixed_kernel(int prec, int contig, int op, int M) {
+    const FixedEntry *e = blue_fixed_entry(prec, contig, op, M);
     return e ? e->fn : nullptr;
 }
 

@@ -44,7 +44,7 @@ struct FixedTable {
 // the Bluestein kernels (dfft_blue.hip): runtime engine of (precision, layout,
 // op); compile-time C2C engine of (precision, layout, convolution length)
 axis_kernel_t blue_kernel(int prec, int contig, int op);
-axis_kernel_t blue_fixed_kernel(int prec, int contig, int M);
+axis_kernel_t blue_fixed_kernel(int prec, int contig, int op, int M);
 
 #define DFFT_FX(T, P, OPV, CONTIG, VAR, EMAXV, SPLIT, RL)                                             \
     {P, OPV, CONTIG, VAR, RL::L, fixed_tpl<RL, EMAXV>(), RL::S, RL::R,                                 \

@@ -18,11 +18,11 @@ void *axis_kernel_fn(const AxisPass &a, int prec);
 bool fixed_schedule(int prec, int contig, int variant, int op, int L, std::vector<int> &radices, int &tpl);
 // every radix list the catalog holds for (precision, layout, variant, op, length)
 std::vector<std::vector<int>> fixed_lists(int prec, int contig, int variant, int op, int L);
-// compile-time C2C Bluestein schedule of convolution length M (a power of
-// two, dfft_blue.hip), if one is built
-bool blue_fixed_schedule(int prec, int contig, int M, std::vector<int> &radices, int &tpl);
+// compile-time Bluestein schedule of (precision, layout, op, convolution
+// length M) (dfft_blue*.hip), if one is built; variant 2 split / 3 full exchange
+bool blue_fixed_schedule(int prec, int contig, int op, int M, std::vector<int> &radices, int &tpl, int &variant);
 // the convolution lengths that have such a schedule
-std::vector<int> blue_fixed_lengths(int prec, int contig);
+std::vector<int> blue_fixed_lengths(int prec, int contig, int op);
 hipError_t launch_fill(void *out, int64_t nscalar, int prec, uint64_t seed, hipStream_t stream);
 hipError_t launch_box(const void *src, int64_t s_off, int64_t s_s1, int64_t s_s2, void *dst, int64_t d_off,
                       int64_t d_s1, int64_t d_s2, const int64_t *cnt, int esize, hipStream_t stream);

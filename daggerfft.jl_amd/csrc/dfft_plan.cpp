@@ -225,15 +225,16 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
     // tile).  Each of its two FFTs costs about (stages x threads per line)
     // barrier-separated LDS rounds, so M is the candidate up to twice the
     // minimum with the least of that product (the smallest M on ties).
-    // A C2C pass takes the compile-time engine instead (both FFTs unrolled,
-    // the exchange between them in registers) over the catalog length
-    // (2^k, 3*2^k, 5*2^k) >= 2n-1 with the least stages x threads per line,
-    // unless the schedule cache names the runtime engine (variant 0).
+    // C2C passes, and the real ops along the contiguous axis, take the
+    // compile-time engine instead (both FFTs unrolled, the exchange between
+    // them in registers) over the catalog length (2^k, 3*2^k, 5*2^k) >= 2n-1
+    // with the least stages x threads per line, unless the schedule cache
+    // names the runtime engine (variant 0).
     a.blue = 0;
     a.bsp = StagePlan{};
     a.bsp2 = StagePlan{};
     std::vector<int> blue_fixed_R;
-    int blue_fixed_tpl = 0;
+    int blue_fixed_tpl = 0, blue_fixed_variant = 0;
     {
         std::vector<int> f;
         if (!factorize(a.sp.L, f)) {
@@ -242,12 +243,12 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
             const bool runtime = tune && tune->variant == 0;
             a.blue = 1;
             int Mf = 0;
-            if (a.op == OP_C2C && !runtime) {
+            if (!runtime) {
                 int64_t best = 0;
-                for (int m : blue_fixed_lengths(pp.prec, a.contig)) {
+                for (int m : blue_fixed_lengths(pp.prec, a.contig, a.op)) {
                     std::vector<int> r;
-                    int t = 0;
-                    if (m < m0 || !blue_fixed_schedule(pp.prec, a.contig, m, r, t)) continue;
+                    int t = 0, var = 0;
+                    if (m < m0 || !blue_fixed_schedule(pp.prec, a.contig, a.op, m, r, t, var)) continue;
                     const int64_t cost = (int64_t)r.size() * t;
                     if (!Mf || cost < best || (cost == best && m < Mf)) {
                         Mf = m;
@@ -256,7 +257,8 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
                 }
             }
             if (Mf) {
-                (void)blue_fixed_schedule(pp.prec, a.contig, Mf, blue_fixed_R, blue_fixed_tpl);
+                (void)blue_fixed_schedule(pp.prec, a.contig, a.op, Mf, blue_fixed_R, blue_fixed_tpl,
+                                          blue_fixed_variant);
                 a.bsp.L = Mf;
             } else {
                 blue_fixed_R.clear();
@@ -305,7 +307,7 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
         }
     }
     if (!blue_fixed_R.empty()) {
-        a.variant = 2;
+        a.variant = blue_fixed_variant;
         R = blue_fixed_R;
         fixed_tpl = blue_fixed_tpl;
     }
@@ -938,7 +940,8 @@ int dfft_plan_describe(const dfft_plan *p, int pass, char *buf, size_t buflen) {
     const char *T = pp.prec ? "double" : "float";
     std::string kname;
     if (a.blue && a.variant)
-        kname = std::string("k_blue_fixed<") + T + "," + (a.contig ? "true" : "false") + ",dfft::Radices<" + rl + ">>";
+        kname = std::string("k_blue_fixed<") + T + "," + std::to_string(a.op) + "," + (a.contig ? "true" : "false") +
+                "," + (a.variant == 2 ? "true" : "false") + ",dfft::Radices<" + rl + ">>";
     else if (a.blue)
         kname = std::string("k_axis_blue<") + T + "," + std::to_string(a.op) + "," + (a.contig ? "true" : "false") + ">";
     else if (a.variant >= 1)

@@ -1,9 +1,10 @@
 """Bluestein (chirp-z) passes: line lengths with a prime factor above 7,
 which FFTW and cuFFT accept and the Stockham catalog cannot factor.  Each
 such axis runs as two M-point FFTs of a cyclic convolution in LDS (M >= 2n-1),
-around the same fused load/store maps as every other pass: C2C passes on the
-compile-time engine (M one of 2^k, 3*2^k, 5*2^k), the others -- and C2C when the
-schedule cache names it -- on the runtime engine (M 7-smooth).
+around the same fused load/store maps as every other pass: C2C passes, and
+R2C/C2R along the contiguous axis, on the compile-time engine (M one of 2^k,
+3*2^k, 5*2^k); the others -- and any of these when the schedule cache names
+it -- on the runtime engine (M 7-smooth).
 
 Parity with the oracle (whose recursive FFT ends in direct DFTs of the prime
 factors, pinned against scipy in test_oracle.py): ComplexF64/Float64 rel-L2
@@ -115,19 +116,27 @@ def test_c2c_compile_time_convolution_lengths(gpu, n, prec):
 
 
 def test_plan_describes_bluestein_passes(gpu):
-    """C2C passes run the compile-time engine over the next power of two;
-    R2C's first pass runs the runtime engine over a 7-smooth length."""
+    """C2C passes, and real ops along the contiguous axis, run the
+    compile-time engine over the cheapest catalog convolution length; the
+    packed R2C op runs its full LDS exchange; a real op along a strided axis
+    runs the runtime engine over a 7-smooth length."""
     import daggerfft as D
     from daggerfft import _lib as L
     p = D.Plan((11, 16, 1009), L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3))
     by_axis = {q["axis"]: q for q in p.passes()}
-    assert by_axis[0]["bluestein"] == 32 and by_axis[0]["kernel"].startswith("k_blue_fixed<double,true")
+    assert by_axis[0]["bluestein"] == 32 and by_axis[0]["kernel"].startswith("k_blue_fixed<double,0,true,true")
     assert D.Plan((257, 2, 2), L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3)).passes()[0]["bluestein"] == 640
     assert by_axis[1]["bluestein"] == 0
-    assert by_axis[2]["bluestein"] == 2048 and by_axis[2]["kernel"].startswith("k_blue_fixed<double,false")
+    assert by_axis[2]["bluestein"] == 2048 and by_axis[2]["kernel"].startswith("k_blue_fixed<double,0,false,true")
     r = D.Plan((22, 16, 8), L.R2C, -1, L.F64, D.Pencil(), (1, 2, 3))
     q = r.passes()[0]
-    assert q["op"] == "R2C" and q["bluestein"] == 21 and q["kernel"].startswith("k_axis_blue<double,1,true")
+    assert q["op"] == "R2C" and q["bluestein"] == 32 and q["kernel"].startswith("k_blue_fixed<double,1,true,false")
+    c = D.Plan((13, 16, 8), L.C2R, +1, L.F32, D.Pencil(), (1, 2, 3))
+    q = [x for x in c.passes() if x["op"] == "C2R_FULL"][0]
+    assert q["kernel"].startswith("k_blue_fixed<float,4,true,true")
+    s = D.Plan((16, 22, 8), L.R2C, -1, L.F64, D.Pencil(), (2, 1, 3))
+    q = s.passes()[0]
+    assert q["op"] == "R2C" and q["axis"] == 1 and q["kernel"].startswith("k_axis_blue<double,1,false")
 
 
 @pytest.mark.parametrize("shape", [(11, 13, 17), (97, 6, 5), (4, 3, 1031), (2003, 2, 2)], ids=lambda s: "x".join(map(str, s)))
