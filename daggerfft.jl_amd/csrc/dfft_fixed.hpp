@@ -88,6 +88,12 @@ axis_kernel_t blue_fixed_kernel(int prec, int contig, int op, int M);
         DFFT_FX(T, P, OP_DCT3, 1, 2, 16, true, RadicesOf16<L>),       \
         DFFT_FX(T, P, OP_DCT3, 0, 2, 16, true, RadicesOf16<L>)
 
+// C2C of a length 1025..4096 that is not a power of two: split exchange,
+// both layouts (RL: the radix list)
+#define DFFT_MID_RL(T, P, RL)                                          \
+    DFFT_FX(T, P, OP_C2C, 1, 2, 16, true, RL), DFFT_FX(T, P, OP_C2C, 0, 2, 16, true, RL)
+#define DFFT_MID(T, P, L) DFFT_MID_RL(T, P, RadicesOf16<L>)
+
 #define DFFT_TABLE(name, ...)                                           \
     static const FixedEntry name##_e[] = {__VA_ARGS__};                 \
     extern const FixedTable name;                                       \

@@ -1,7 +1,8 @@
-"""Long lines: the power-of-two lengths 2048..16384 run compile-time
-catalog kernels (16 values per thread, split-LDS exchange, dfft_fixed_long.hip)
-along the contiguous axis, a strided axis with outer slices and a strided
-axis without, in both precisions; FP64 4096 through both of its radix lists.
+"""Long lines: the power-of-two lengths 2048..16384 (dfft_fixed_long.hip) and
+every {2,3,5,7}-smooth length 1025..4096 (dfft_fixed_mid*.hip) run
+compile-time catalog kernels (16 values per thread, split-LDS exchange) along
+the contiguous axis, a strided axis with outer slices and a strided axis
+without, in both precisions; FP64 4096 through both of its radix lists.
 ComplexF64 rel-L2 <= 1e-12; ComplexF32 against the FP64 oracle <= 2e-6."""
 import math
 import os
@@ -14,6 +15,16 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 TOL = {1: 1e-12, 0: 2e-6}
 LONG = [2048, 4096, 8192, 16384]
+
+
+def _smooth(m):
+    for p in (2, 3, 5, 7):
+        while m % p == 0:
+            m //= p
+    return m == 1
+
+
+MID = [m for m in range(1025, 4097) if _smooth(m) and m & (m - 1)]
 
 
 def _case(D, L, shape, prec, ax):
@@ -42,6 +53,23 @@ def test_long_lines_catalog(gpu, n, prec):
         assert e <= TOL[prec] and e2 <= TOL[prec], (shape, e, e2)
     if prec == 1 and n == 4096:
         assert q["radices"] == "8x8x8x8", q  # the default list of FP64 4096 (16x16x16 spills)
+
+
+@pytest.mark.parametrize("prec", [1, 0], ids=["f64", "f32"])
+@pytest.mark.parametrize("part", range(4))
+def test_mid_lengths_catalog(gpu, prec, part):
+    import daggerfft as D
+    from daggerfft import _lib as L
+    D.tune_clear()
+    worst = 0.0
+    for n in MID[part::4]:
+        for shape, ax in (((n, 3, 2), 0), ((3, n, 2), 1), ((2, 3, n), 2)):
+            print("mid", n, "f64" if prec else "f32", shape, flush=True)
+            q, e, e2 = _case(D, L, shape, prec, ax)
+            assert q["variant"] == 2 and q["L"] == n, q
+            assert e <= TOL[prec] and e2 <= TOL[prec], (shape, e, e2)
+            worst = max(worst, e, e2)
+    print("worst rel-L2 (%s): %.3e" % ("f64" if prec else "f32", worst))
 
 
 def test_f64_4096_both_radix_lists(gpu, tmp_path):
