@@ -79,6 +79,11 @@ axis_kernel_t blue_fixed_kernel(int prec, int contig, int op, int M);
         DFFT_FX(double, 1, OP_C2R, 1, 2, 16, true, RadicesOf16<H>), \
         DFFT_FX(float, 0, OP_C2R, 1, 2, 16, true, RadicesOf16<H>)
 
+// C2R with the split exchange only (the long half-lengths)
+#define DFFT_C2R_SPLIT(H)                                              \
+    DFFT_FX(double, 1, OP_C2R, 1, 2, 16, true, RadicesOf16<H>),     \
+        DFFT_FX(float, 0, OP_C2R, 1, 2, 16, true, RadicesOf16<H>)
+
 // DCT-II / DCT-III (REDFT10 / REDFT01, Makhoul's n-point algorithm) of
 // length L, split LDS exchange, both layouts: their fused load and store maps
 // touch only global memory, so the split exchange applies

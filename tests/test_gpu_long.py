@@ -2,7 +2,10 @@
 every {2,3,5,7}-smooth length 1025..4096 (dfft_fixed_mid*.hip) run
 compile-time catalog kernels (16 values per thread, split-LDS exchange) along
 the contiguous axis, a strided axis with outer slices and a strided axis
-without, in both precisions; FP64 4096 through both of its radix lists.
+without, in both precisions; FP64 4096 through both of its radix lists.  The
+fused C2R (split exchange) of every smooth half-length 257..1024
+(dfft_fixed_real*.hip) likewise, with the R2C of the same lines (runtime
+engine).
 ComplexF64 rel-L2 <= 1e-12; ComplexF32 against the FP64 oracle <= 2e-6."""
 import math
 import os
@@ -25,6 +28,7 @@ def _smooth(m):
 
 
 MID = [m for m in range(1025, 4097) if _smooth(m) and m & (m - 1)]
+HALF_LONG = [h for h in range(257, 1025) if _smooth(h)]
 
 
 def _case(D, L, shape, prec, ax):
@@ -69,6 +73,34 @@ def test_mid_lengths_catalog(gpu, prec, part):
             assert q["variant"] == 2 and q["L"] == n, q
             assert e <= TOL[prec] and e2 <= TOL[prec], (shape, e, e2)
             worst = max(worst, e, e2)
+    print("worst rel-L2 (%s): %.3e" % ("f64" if prec else "f32", worst))
+
+
+@pytest.mark.parametrize("prec", [1, 0], ids=["f64", "f32"])
+def test_long_half_lengths_r2c_c2r(gpu, prec):
+    import daggerfft as D
+    from daggerfft import _lib as L
+    D.tune_clear()
+    rdt = np.float64 if prec else np.float32
+    cdt = np.complex128 if prec else np.complex64
+    worst = 0.0
+    for h in HALF_LONG:
+        shape = (2 * h, 3, 2)
+        print("R2C/C2R", h, "f64" if prec else "f32", flush=True)
+        a = O.fill(h, shape, rdt)
+        ref = O.rfftn(a.astype(np.float64))
+        pr = D.Plan(shape, L.R2C, -1, prec, D.Pencil(), (1, 2, 3))
+        q = pr.passes()[0]
+        assert q["op"] == "R2C", q
+        F = pr.execute(D.DeviceArray.from_host(a), D.DeviceArray((h + 1, 3, 2), cdt))
+        e = O.rel_l2(F.to_host().astype(np.complex128), ref)
+        pc = D.Plan(shape, L.C2R, +1, prec, D.Pencil(), (1, 2, 3))
+        qc = [x for x in pc.passes() if x["op"] == "C2R"][0]
+        assert qc["variant"] == 2, qc
+        B = pc.execute(D.DeviceArray.from_host(ref.astype(cdt)), D.DeviceArray(shape, rdt))
+        e2 = O.rel_l2(B.to_host().astype(np.float64), a.astype(np.float64))
+        assert e <= TOL[prec] and e2 <= TOL[prec], (h, e, e2)
+        worst = max(worst, e, e2)
     print("worst rel-L2 (%s): %.3e" % ("f64" if prec else "f32", worst))
 
 
