@@ -113,7 +113,7 @@ def test_dct_vs_scipy(shape):
 
 @pytest.mark.parametrize("kind,ref", [("REDFT00", ("dct", 1)), ("REDFT11", ("dct", 4)), ("RODFT00", ("dst", 1)),
                                       ("RODFT10", ("dst", 2)), ("RODFT01", ("dst", 3)), ("RODFT11", ("dst", 4))])
-@pytest.mark.parametrize("shape", [(9, 6, 5), (7, 5, 9), (12, 2, 3)])
+@pytest.mark.parametrize("shape", [(9, 6, 5), (7, 5, 9), (12, 2, 3), (11, 13, 2), (97, 3, 2)])
 def test_r2r_kinds_vs_scipy(kind, ref, shape):
     """The remaining FFTW kinds against pocketfft's DCT/DST types 1-4
     (norm=None uses FFTW's unnormalised definitions)."""
