@@ -102,18 +102,17 @@ static axis_kernel_t axis_kernel(const AxisPass &a, int prec) {
     return g_axis_kernels[prec ? 1 : 0][a.contig ? 1 : 0][a.op];
 }
 
-static int g_lds_limit = 0;
-
+// LDS per workgroup of the device: read once (a function-local static is
+// initialised once even when several threads create plans at the same time)
 int axis_lds_limit() {
-    if (g_lds_limit == 0) {
+    static const int limit = [] {
         int dev = 0, v = 0;
         if (hipGetDevice(&dev) == hipSuccess &&
             hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
-            g_lds_limit = v;
-        else
-            g_lds_limit = 65536;
-    }
-    return g_lds_limit;
+            return v;
+        return 65536;
+    }();
+    return limit;
 }
 
 hipError_t launch_axis(const AxisPass &a, int prec, int64_t nblocks, int threads, size_t lds,

@@ -5,7 +5,9 @@
  * Every entry point returns an int status (DFFT_OK == 0); nothing throws or
  * aborts across this boundary.  All device work is enqueued on the caller's
  * HIP stream (passed as void*; NULL = the default stream) and nothing
- * synchronises the device implicitly except where stated.
+ * synchronises the device implicitly except where stated.  Plans may be
+ * created and destroyed from several threads at once; one plan may execute
+ * concurrently on distinct streams with distinct workspaces.
  *
  * Layout: arrays are column-major (Julia order).  An array of size
  * n[0] x n[1] x n[2] holds element (i,j,k) at i + n[0]*(j + n[1]*k); complex
