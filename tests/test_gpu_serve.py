@@ -64,6 +64,36 @@ def test_served_mix_equals_alone(gpu):
         assert e <= (1e-12 if q.precision == 1 else 1e-5), (q, e)
 
 
+def test_served_unsnapped_mix_with_bluestein(gpu):
+    """The mix with edges rounded to integers (not snapped to 7-smooth
+    sizes): requests whose edge has a prime factor above 7 run Bluestein
+    passes; served results equal the same transforms alone, and the oracle on
+    the smallest."""
+    from daggerfft import serve as S
+    trace = S.serving_trace(count=24, seed=5, rate=4000.0, lo=16, hi=96, snap=False)
+    edges = [q.shape[0] for q in trace]
+    print("mix:", [(e, q.kind, q.precision) for e, q in zip(edges, trace)], flush=True)
+    assert any(e % p == 0 for e in edges for p in (11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47))
+    with S.Server(hbm_budget=4 * S.GiB, max_inflight=8, lanes=4) as srv:
+        tickets = [srv.submit(q) for q in trace]
+        got = []
+        for t, q in zip(tickets, trace):
+            assert srv.wait(t, 60.0)
+            assert srv.info(t)["status"] == 0
+            got.append(srv.result(t, q).to_host())
+            srv.release(t)
+        st = srv.stats()
+    assert st["completed"] == len(trace) and st["failed"] == 0
+    for q, y in zip(trace, got):
+        a = _alone(q)
+        assert O.rel_l2(y.astype(np.complex128), a.astype(np.complex128)) <= TOL[q.precision], q
+    for q, y in sorted(zip(trace, got), key=lambda p: p[0].shape[0])[:4]:
+        x = O.fill(q.seed, q.shape, np.dtype(q.in_dtype)).astype(
+            np.float64 if q.kind == 1 else np.complex128)
+        ref = O.rfftn(x) if q.kind == 1 else O.fftn(x)
+        assert O.rel_l2(y.astype(np.complex128), ref) <= (1e-12 if q.precision == 1 else 1e-5), q
+
+
 def test_open_loop_replay_latency_and_results(gpu):
     from daggerfft import serve as S
     trace = _mix(seed=5, count=40, lo=16, hi=128)
