@@ -230,3 +230,23 @@ def test_lengths_beyond_one_tile_are_refused(gpu):
     with pytest.raises(D.DFFTError) as ei:
         D.Plan((4099, 2, 1), L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3))
     assert ei.value.status == L.DFFT_ERR_UNSUPPORTED_SIZE
+
+
+def test_autotune_skips_bluestein_passes(gpu):
+    """The autotuner has no candidates for a Bluestein pass: tuning a shape
+    with prime axes records entries only for its smooth axes, and the plan
+    afterwards still runs Bluestein passes and matches the oracle."""
+    import daggerfft as D
+    from daggerfft import _lib as L
+    shape = (11, 64, 13)
+    try:
+        D.tune_clear()
+        D.autotune(shape, L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3), level=1)
+        assert D.tune_entries() == 1  # the 64-point axis only
+        p = D.Plan(shape, L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3))
+        assert [q["bluestein"] > 0 for q in p.passes() if q["axis"] != 1] == [True, True]
+        a = O.fill(27, shape, np.complex128)
+        F = p.execute(_dev(a), D.DeviceArray(shape, np.complex128))
+        assert O.rel_l2(F.to_host(), O.fftn(a)) <= TOL64
+    finally:
+        D.tune_clear()
