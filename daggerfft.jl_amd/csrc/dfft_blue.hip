@@ -46,7 +46,7 @@ axis_kernel_t blue_kernel(int prec, int contig, int op) {
 // C2C of convolution lengths 2^k (32..8192) here, 3*2^k and 5*2^k in
 // dfft_blue_fixed2.hip, the real ops in dfft_blue_real64/32.hip
 DFFT_TABLE(g_blue_pow2, DFFT_BLUE_M(32), DFFT_BLUE_M(64), DFFT_BLUE_M(128), DFFT_BLUE_M(256), DFFT_BLUE_M(512),
-           DFFT_BLUE_M(1024), DFFT_BLUE_M(2048), DFFT_BLUE_M(4096), DFFT_BLUE_M(8192))
+           DFFT_BLUE_M(1024), DFFT_BLUE_M(2048), DFFT_BLUE_M(4096), DFFT_BLUE_M(8192), DFFT_BLUE_M(16384))
 extern const FixedTable g_blue_mixed, g_blue_real64, g_blue_real32;
 
 static const FixedEntry *blue_fixed_entry(int prec, int contig, int op, int M) {

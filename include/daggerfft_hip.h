@@ -48,7 +48,7 @@ extern "C" {
 /* ---- status codes ------------------------------------------------------ */
 #define DFFT_OK                    0
 #define DFFT_ERR_ARG               1  /* bad argument (ArgumentError in the reference) */
-#define DFFT_ERR_UNSUPPORTED_SIZE  2  /* a line too long for one LDS tile (n > 8192 except 16384, or n > 4096 with a prime factor > 7) */
+#define DFFT_ERR_UNSUPPORTED_SIZE  2  /* a line too long for one LDS tile: n > 8192 except 16384; with a prime factor > 7, n > 8192 (C2C) or n > 4096 (other ops) */
 #define DFFT_ERR_HIP               3  /* a HIP runtime call failed */
 #define DFFT_ERR_OOM               4  /* device allocation failed */
 #define DFFT_ERR_BUDGET            5  /* serving admission: request exceeds the HBM budget */

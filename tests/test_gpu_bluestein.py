@@ -222,13 +222,23 @@ def test_batch_with_prime_lengths(gpu):
 
 def test_lengths_beyond_one_tile_are_refused(gpu):
     """Bluestein needs M >= 2n-1 points of one line in LDS with at most 1024
-    threads: n = 4093 runs (M = 8192), 4099 (prime) does not (every 7-smooth
-    M >= 8197 needs more than 1024 threads per line)."""
+    threads.  C2C runs up to n = 8192 (the compile-time engine's M = 16384):
+    8191 (prime) runs and matches the oracle, 8209 (prime) is refused.  The
+    runtime engine (the other ops) takes M <= 8192: R2C of 2 x 4099 points
+    (a 4099-point half-length) is refused."""
     import daggerfft as D
     from daggerfft import _lib as L
-    D.Plan((4096 - 3, 2, 1), L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3))
+    a = O.fill(28, (8191, 2, 1), np.complex128)
+    p = D.Plan(a.shape, L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3))
+    assert p.passes()[0]["bluestein"] == 16384
+    F = p.execute(_dev(a), D.DeviceArray(a.shape, np.complex128))
+    assert O.rel_l2(F.to_host(), O.fftn(a)) <= TOL64
     with pytest.raises(D.DFFTError) as ei:
-        D.Plan((4099, 2, 1), L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3))
+        D.Plan((8209, 2, 1), L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3))
+    assert ei.value.status == L.DFFT_ERR_UNSUPPORTED_SIZE
+    D.Plan((4093 * 2, 2, 1), L.R2C, -1, L.F64, D.Pencil(), (1, 2, 3))
+    with pytest.raises(D.DFFTError) as ei:
+        D.Plan((4099 * 2, 2, 1), L.R2C, -1, L.F64, D.Pencil(), (1, 2, 3))
     assert ei.value.status == L.DFFT_ERR_UNSUPPORTED_SIZE
 
 
