@@ -43,3 +43,17 @@ def test_every_measured_pass_schedules(checker, wisdom):
     print(r.stdout[-2000:])
     assert r.returncode == 0, r.stdout[-2000:]
     assert "bad 0" in r.stdout
+
+
+def test_bluestein_choices_on_the_host(tmp_path):
+    """Bluestein passes are chosen exactly for lengths with a prime factor
+    above 7, with a valid convolution length and schedule (tests/bluestein_check.cpp)."""
+    if shutil.which("g++") is None or not os.path.exists(os.path.join(LIBDIR, "libdaggerfft_hip.so")):
+        pytest.skip("needs g++ and the built library")
+    exe = str(tmp_path / "bluestein_check")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"),
+                    "-I", os.path.join(LIBDIR, "csrc"), os.path.join(ROOT, "tests", "bluestein_check.cpp"),
+                    "-o", exe, "-L", LIBDIR, "-ldaggerfft_hip", "-Wl,-rpath," + LIBDIR], check=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:])
+    assert r.returncode == 0, r.stdout[-2000:]
