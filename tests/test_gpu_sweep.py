@@ -2,8 +2,8 @@
 with unit and odd edges, random dims subsets and orders, C2C forward and
 inverse, R2C, C2R and every r2r kind of R2R_SUPPORTED_KINDS, both precisions,
 Pencil and Slab.  The first set draws 7-smooth edges (Stockham passes), the
-second any edge up to 140 (primes and other lengths with a factor above 7 run
-Bluestein passes).  ComplexF64/Float64 rel-L2 <= 1e-12; ComplexF32/Float32
+second any edge up to 140 and the third any edge up to 300 (primes and other
+lengths with a factor above 7 run Bluestein passes).  ComplexF64/Float64 rel-L2 <= 1e-12; ComplexF32/Float32
 against the FP64 oracle <= 2e-5.  The only refusal is REDFT00 of length 1,
 which FFTW rejects too."""
 import numpy as np
@@ -26,6 +26,7 @@ def _smooth(m):
 
 
 ANY = list(range(1, 141))
+ANY300 = list(range(1, 301))
 
 
 def _r2r_ok(kind, n):
@@ -53,7 +54,7 @@ def _cases(count=200, seed=2024, sizes=SMOOTH, first=0):
     return out
 
 
-@pytest.mark.parametrize("case", _cases() + _cases(120, 4049, ANY, 200), ids=lambda c: "%d-%s-%s-%s-f%d-%s" % (
+@pytest.mark.parametrize("case", _cases() + _cases(120, 4049, ANY, 200) + _cases(150, 777, ANY300, 320), ids=lambda c: "%d-%s-%s-%s-f%d-%s" % (
     c[0], "x".join(map(str, c[1])), "".join(map(str, c[2])), c[3], 64 if c[4] else 32, c[5]))
 def test_sweep_against_oracle(gpu, case):
     import daggerfft as D
