@@ -61,6 +61,22 @@ def test_chunked_pencil_fft_ifft(gpu):
     assert O.rel_l2(back.collect().to_host(), a) <= 1e-12
 
 
+def test_chunked_pencil_prime_edges(gpu):
+    """Chunked arrays with prime edges (Bluestein passes inside each chunk's
+    apply_fft!) and uneven blocks."""
+    import daggerfft as D
+    shape, ch = (23, 19, 13), 7
+    a = O.fill(44, shape, np.complex128)
+    A = D.DArray.distribute(D.DeviceArray.from_host(a), (shape[0], ch, ch))
+    B = D.DArray(shape, (ch, shape[1], ch), np.complex128)
+    Cz = D.DArray(shape, (ch, ch, shape[2]), np.complex128)
+    out = D.fft_(Cz, A, B, D.create_workspace(A, B), D.create_workspace(B, Cz), (D.FFT_(), D.FFT_(), D.FFT_()),
+                 (1, 2, 3))
+    F = out.collect().to_host()
+    assert O.rel_l2(F, O.fft_pencil(a, chunk=ch)) <= 1e-12
+    assert O.rel_l2(F, O.fftn(a)) <= 1e-12
+
+
 def test_chunked_slab_fft_ifft(gpu):
     import daggerfft as D
     a = O.fill(43, SHAPE, np.complex128)
