@@ -260,3 +260,19 @@ def test_autotune_skips_bluestein_passes(gpu):
         assert O.rel_l2(F.to_host(), O.fftn(a)) <= TOL64
     finally:
         D.tune_clear()
+
+
+@pytest.mark.parametrize("shape,prec", [((8191, 2, 1), 0), ((2, 8191, 1), 1), ((2, 8191, 1), 0), ((2, 1, 8191), 1)],
+                         ids=["contig-f32", "strided-f64", "strided-f32", "plane-f64"])
+def test_longest_bluestein_lines_every_layout(gpu, shape, prec):
+    """The 16384-point convolution (n = 8191, prime) in both precisions,
+    along the contiguous axis and both strided layouts."""
+    import daggerfft as D
+    from daggerfft import _lib as L
+    dt = np.complex128 if prec else np.complex64
+    a = O.fill(29, shape, dt)
+    p = D.Plan(shape, L.C2C, -1, prec, D.Pencil(), (1, 2, 3))
+    q = [x for x in p.passes() if x["L"] == 8191][0]
+    assert q["bluestein"] == 16384 and q["kernel"].startswith("k_blue_fixed"), q
+    F = p.execute(_dev(a), D.DeviceArray(shape, dt))
+    assert O.rel_l2(F.to_host().astype(np.complex128), O.fftn(a.astype(np.complex128))) <= (TOL64 if prec else TOL32)
