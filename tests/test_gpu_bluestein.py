@@ -276,3 +276,25 @@ def test_longest_bluestein_lines_every_layout(gpu, shape, prec):
     assert q["bluestein"] == 16384 and q["kernel"].startswith("k_blue_fixed"), q
     F = p.execute(_dev(a), D.DeviceArray(shape, dt))
     assert O.rel_l2(F.to_host().astype(np.complex128), O.fftn(a.astype(np.complex128))) <= (TOL64 if prec else TOL32)
+
+
+@pytest.mark.parametrize("n", [2 * 4093, 4093], ids=["packed-H4093", "full-4093"])
+@pytest.mark.parametrize("prec", [1, 0], ids=["f64", "f32"])
+def test_longest_real_bluestein_lines(gpu, n, prec):
+    """rfft / irfft of the longest real lines with a prime factor above 7:
+    packed R2C/C2R of half-length 4093 and R2C_FULL/C2R_FULL of length 4093,
+    both on 8192-point convolutions."""
+    import daggerfft as D
+    from daggerfft import _lib as L
+    rdt = np.float64 if prec else np.float32
+    cdt = np.complex128 if prec else np.complex64
+    tol = TOL64 if prec else TOL32
+    shape = (n, 2, 1)
+    x = O.fill(30, shape, rdt)
+    pr = D.Plan(shape, L.R2C, -1, prec, D.Pencil(), (1, 2, 3))
+    assert pr.passes()[0]["bluestein"] == 8192, pr.passes()[0]
+    F = pr.execute(_dev(x), D.DeviceArray((n // 2 + 1, 2, 1), cdt))
+    ref = O.rfftn(x.astype(np.float64))
+    assert O.rel_l2(F.to_host().astype(np.complex128), ref) <= tol
+    y = D.irfft(_dev(ref.astype(cdt)), n, decomp=D.Pencil(), dims=(1, 2, 3))
+    assert O.rel_l2(y.to_host().astype(np.float64), x.astype(np.float64)) <= tol
