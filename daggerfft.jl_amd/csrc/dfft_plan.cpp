@@ -536,7 +536,10 @@ int build_plan(dfft_plan *p) {
         // The 1D transforms along distinct axes commute.  The first pass is the
         // out-of-place one, and out-of-place is cheapest on the contiguous axis,
         // so it goes first unless the reference's stage order is requested.
-        if (!(p->flags & DFFT_FLAG_STAGE_ORDER)) {
+        // The Slab plane pipeline keeps the stage order: it chunks the two
+        // in-plane passes over z-planes and needs the dims[2] pass at one end.
+        const bool plane_pipeline = p->decomp == DFFT_SLAB && (p->flags & DFFT_FLAG_PLANE_PIPELINE);
+        if (!(p->flags & DFFT_FLAG_STAGE_ORDER) && !plane_pipeline) {
             auto it = std::find(order.begin(), order.end(), 0);
             if (it != order.end()) std::rotate(order.begin(), it, it + 1);
         }
@@ -636,6 +639,13 @@ int build_plan(dfft_plan *p) {
     if (slab_ok) {
         const bool inv = p->sign > 0;
         const int ia = inv ? 1 : 0, ib = inv ? 2 : 1;
+        // The line pass must be the z (axis 2) pass at the pipeline's end that
+        // reads (inverse) or writes (forward) the whole array, and the two
+        // chunked passes must stay inside z-planes; anything else would make a
+        // chunk address planes outside its own.
+        if (p->passes.size() != 3 || p->passes[inv ? 0 : 2].axis != 2 || p->passes[ia].axis == 2 ||
+            p->passes[ib].axis == 2)
+            return DFFT_ERR_UNSUPPORTED;
         line = p->passes[inv ? 0 : 2];
         {
             const Tuning *tune = nullptr;
@@ -818,7 +828,8 @@ int dfft_plan_create(dfft_plan **plan, int rank, const int64_t *n, int kind, int
     }
     p->rank = rank;
     for (int d = 0; d < rank; ++d) {
-        if (n[d] < 1 || n[d] > (1LL << 31)) { delete p; return DFFT_ERR_ARG; }
+        // every length must fit an int (line lengths are int on the device)
+        if (n[d] < 1 || n[d] >= (1LL << 31)) { delete p; return DFFT_ERR_ARG; }
         p->n[d] = n[d];
     }
     p->kind = kind;

@@ -196,6 +196,21 @@ def _stream_handle(stream=None):
     return ctypes.c_void_p(s.cuda_stream)
 
 
+class _NoStream:
+    def __enter__(self):
+        return None
+
+    def __exit__(self, *exc):
+        return False
+
+
+def _on_stream(stream):
+    """Context in which torch allocations and conversions run on `stream`
+    (torch's current stream when None), so they are ordered with the
+    library's launches on that stream."""
+    return _torch().cuda.stream(stream) if stream is not None else _NoStream()
+
+
 def _splitmix_u(seed, i):
     """u(seed, i) of the SplitMix64 counter generator, on the host."""
     M = (1 << 64) - 1
@@ -270,10 +285,18 @@ class Plan:
         return b.value
 
     def execute(self, inp, out, work=None, stream=None):
-        """Enqueue the plan on `stream` (default: torch's current stream)."""
+        """Enqueue the plan on `stream` (default: torch's current stream).
+
+        A plan that needs a workspace (C2R without DFFT_FLAG_DESTROY_INPUT)
+        gets a fresh one per call from torch's stream-ordered allocator when
+        `work` is None, so concurrent executions of one cached plan on
+        different streams or threads never share scratch memory."""
         if inp.nbytes != self.in_bytes or out.nbytes != self.out_bytes:
             raise ValueError("array sizes do not match the plan (%d/%d vs %d/%d bytes)"
                              % (inp.nbytes, out.nbytes, self.in_bytes, self.out_bytes))
+        if work is None and self.workspace_bytes:
+            with _on_stream(stream):
+                work = _torch().empty(self.workspace_bytes, dtype=_torch().uint8, device=out.tensor.device)
         wptr = None
         if work is not None:
             wptr = work.data_ptr() if hasattr(work, "data_ptr") else int(work)
@@ -282,7 +305,9 @@ class Plan:
         return out
 
     def execute_pass(self, i, inp, out, work=None, stream=None):
-        """Enqueue axis pass `i` alone (passes 0..n-1 in order == execute)."""
+        """Enqueue axis pass `i` alone (passes 0..n-1 in order == execute).
+        Without `work`, a plan that needs a workspace uses its own one, so the
+        passes of one execution see the same intermediate buffer."""
         wptr = None
         if work is not None:
             wptr = work.data_ptr() if hasattr(work, "data_ptr") else int(work)
@@ -430,15 +455,16 @@ def fft(A, decomp=Pencil(), dims=None, transforms=None, stream=None):
     """F = fft(A; decomp, dims[, transforms=(R2R(kinds),)]) -- unnormalised forward."""
     dims = _dims(A, dims)
     kinds = _r2r_kinds(transforms, dims, decomp)
-    if kinds is not None:
-        if A.is_complex:
-            raise TypeError("R2R transforms need a real array")
-        p = plan_r2r(A, kinds, dims, decomp, sign=-1)
+    with _on_stream(stream):
+        if kinds is not None:
+            if A.is_complex:
+                raise TypeError("R2R transforms need a real array")
+            p = plan_r2r(A, kinds, dims, decomp, sign=-1)
+            return p.execute(A, A.similar(), stream=stream)
+        if not A.is_complex:
+            A = _to_complex(A)
+        p = plan_fft(A, dims, decomp)
         return p.execute(A, A.similar(), stream=stream)
-    if not A.is_complex:
-        A = _to_complex(A, stream)
-    p = plan_fft(A, dims, decomp)
-    return p.execute(A, A.similar(), stream=stream)
 
 
 def ifft(F, decomp=Pencil(), dims=None, transforms=None, stream=None):
@@ -447,13 +473,14 @@ def ifft(F, decomp=Pencil(), dims=None, transforms=None, stream=None):
     REDFT01 ifft of an REDFT10 fft reconstructs the input."""
     dims = _dims(F, dims)
     kinds = _r2r_kinds(transforms, dims, decomp)
-    if kinds is not None:
-        p = plan_r2r(F, kinds, dims, decomp, sign=+1, flags=L.FLAG_NORMALIZE_R2R)
+    with _on_stream(stream):
+        if kinds is not None:
+            p = plan_r2r(F, kinds, dims, decomp, sign=+1, flags=L.FLAG_NORMALIZE_R2R)
+            return p.execute(F, F.similar(), stream=stream)
+        if not F.is_complex:
+            F = _to_complex(F)
+        p = plan_ifft(F, dims, decomp)
         return p.execute(F, F.similar(), stream=stream)
-    if not F.is_complex:
-        F = _to_complex(F, stream)
-    p = plan_ifft(F, dims, decomp)
-    return p.execute(F, F.similar(), stream=stream)
 
 
 def rfft(A, decomp=Pencil(), dims=None, stream=None):
@@ -464,7 +491,9 @@ def rfft(A, decomp=Pencil(), dims=None, stream=None):
     p = plan_rfft(A, dims, decomp)
     shape = list(A.shape)
     shape[dims[0] - 1] = shape[dims[0] - 1] // 2 + 1
-    return p.execute(A, DeviceArray(shape, _complex_of(A.dtype), device=A.tensor.device), stream=stream)
+    with _on_stream(stream):
+        out = DeviceArray(shape, _complex_of(A.dtype), device=A.tensor.device)
+        return p.execute(A, out, stream=stream)
 
 
 def irfft(F, d, decomp=Pencil(), dims=None, stream=None):
@@ -473,11 +502,14 @@ def irfft(F, d, decomp=Pencil(), dims=None, stream=None):
         raise TypeError("irfft needs a complex array")
     dims = _dims(F, dims)
     p = plan_irfft(F, d, dims, decomp)
-    return p.execute(F, DeviceArray(p.shape, _real_of(F.dtype), device=F.tensor.device), stream=stream)
+    with _on_stream(stream):
+        out = DeviceArray(p.shape, _real_of(F.dtype), device=F.tensor.device)
+        return p.execute(F, out, stream=stream)
 
 
-def _to_complex(A, stream=None):
-    torch = _torch()
+def _to_complex(A):
+    """Complex copy of a real array, made on torch's current stream (callers
+    enter the execution stream first)."""
     t = A.tensor.to(_torch_dtype(_complex_of(A.dtype)))
     return DeviceArray(A.shape, _complex_of(A.dtype), t)
 
@@ -515,18 +547,96 @@ def cleanup_workspace(ws):
 gpu_cleanup_workspace = cleanup_workspace
 
 
-def _stage_plan(A, transforms, dims, decomp, inverse):
-    kinds = _r2r_kinds(transforms, dims, decomp)
-    if kinds is not None:
-        return plan_r2r(A, kinds, dims, decomp, sign=+1 if inverse else -1)
-    t0 = transforms[0] if isinstance(transforms, (tuple, list)) else transforms
-    inv = isinstance(t0, (IFFT, IFFT_)) if t0 is not None else inverse
-    return (plan_ifft if inv else plan_fft)(A, dims, decomp)
+def _stage_list(transforms, dims, decomp, inverse):
+    """The (transform, dims) of each stage in execution order.
+
+    Pencil fft! runs transforms[i] along dims[i] for i = 1, 2, 3 and ifft!
+    runs them in the order 3, 2, 1 (src/fft.jl:651-724).  Slab runs
+    transforms[1] over (dims[1], dims[2]) and transforms[3] over dims[3],
+    ignoring transforms[2] (src/fft.jl:727-776).  A single transform (not a
+    tuple) applies to every stage; a one-element tuple holding an R2R with one
+    kind per dim is the README form (README.md:59-66)."""
+    ts = tuple(transforms) if isinstance(transforms, (tuple, list)) else None
+    dims = tuple(int(d) for d in dims)
+    slab = isinstance(decomp, Slab)
+    groups = [(dims[0], dims[1]), (dims[2],)] if slab else [(d,) for d in dims]
+    if ts is None:
+        per = [transforms] * len(groups)
+    elif len(ts) == 1 and isinstance(ts[0], R2R) and len(ts[0].kinds) == len(dims) > 1:
+        ks, per, i = ts[0].kinds, [], 0
+        for g in groups:
+            per.append(R2R(ks[i:i + len(g)]))
+            i += len(g)
+    elif len(ts) == 1:
+        per = [ts[0]] * len(groups)
+    else:
+        if len(ts) < len(dims):
+            raise ValueError("transforms needs one entry per stage (%d given, %d dims)" % (len(ts), len(dims)))
+        per = [ts[0], ts[2]] if slab else list(ts[:len(dims)])
+    stages = list(zip(per, groups))
+    return stages[::-1] if inverse else stages
+
+
+def _stage_op(t, ndims):
+    """('c2c', sign, None) or ('r2r', -1, kinds) for one stage (plan_transform,
+    src/fft.jl:126-144: FFT/FFT! forward, IFFT/IFFT! inverse scaled by 1/n per
+    dim).  RFFT!/IRFFT! change the element type, so an in-place stage cannot
+    hold them (the reference's plan_rfft! does not exist in FFTW.jl): they
+    raise, as the reference's call would."""
+    if isinstance(t, R2R):
+        kinds = t.kinds if len(t.kinds) == ndims else t.kinds[:1] * ndims
+        return "r2r", -1, tuple(kinds)
+    if isinstance(t, (FFT, FFT_)):
+        return "c2c", -1, None
+    if isinstance(t, (IFFT, IFFT_)):
+        return "c2c", +1, None
+    if isinstance(t, (RFFT, RFFT_, IRFFT, IRFFT_)):
+        raise ValueError("%r changes the element type and cannot run as an in-place stage; "
+                         "use rfft/irfft" % (t,))
+    raise ValueError("Unknown transform type %r" % (t,))
+
+
+def _run_stages(src, out, transforms, dims, decomp, inverse, stream):
+    """Run the bang API's stages from `src` into `out` on one device block.
+
+    Stages of one kind and direction form one plan (one launch per axis).
+    Mixed tuples, e.g. (FFT!(), IFFT!(), FFT!()), run one plan per stage:
+    the first stage out of place from `src`, the rest in place on `out`."""
+    stages = [(_stage_op(t, len(g)), g) for t, g in _stage_list(transforms, dims, decomp, inverse)]
+    kinds = {(op, sign) for (op, sign, _), _ in stages}
+    if len(kinds) == 1:
+        (op, sign), = kinds
+        fwd = stages[::-1] if inverse else stages  # the plan takes dims in forward stage order
+        all_dims = tuple(d for _, g in fwd for d in g)
+        if op == "c2c":
+            if not src.is_complex:
+                raise TypeError("FFT!/IFFT! stages need a complex array")
+            p = (plan_ifft if (sign > 0) else plan_fft)(src, all_dims, decomp)
+        else:
+            if src.is_complex:
+                raise TypeError("R2R stages need a real array")
+            ks = tuple(k for (_, _, kk), _ in fwd for k in kk)
+            p = plan_r2r(src, ks, all_dims, decomp, sign=+1 if inverse else -1)
+        return p.execute(src, out, stream=stream)
+    cur = src
+    for (op, sign, kk), g in stages:
+        if op == "c2c":
+            if not src.is_complex:
+                raise TypeError("FFT!/IFFT! stages need a complex array")
+            p = _cached_plan(src.shape, L.C2C, sign, src.precision, Pencil(), g)
+        else:
+            if src.is_complex:
+                raise TypeError("R2R stages need a real array")
+            p = _cached_plan(src.shape, L.R2R, -1, src.precision, Pencil(), g, kk)
+        p.execute(cur, out, stream=stream)
+        cur = out
+    return out
 
 
 def fft_(*args, stream=None):
     """fft!(C, A, B, ws_AB, ws_BC, transforms, dims) [Pencil] or
-    fft!(B, A, ws, transforms, dims, Slab()).  Returns the output array; the
+    fft!(B, A, ws, transforms, dims, Slab()).  Stage i applies transforms[i]
+    along dims[i] (src/fft.jl:651-688, 727-750).  Returns the output array; the
     stage array A is the input (the reference also leaves stage results in
     A/B, which this single-chunk version does not).  Chunked arrays
     (chunked.DArray) take the reference's per-chunk stages and box-copy
@@ -535,29 +645,26 @@ def fft_(*args, stream=None):
         return chunked.fft_(*args, stream=stream)
     if len(args) == 7:
         C, A, B, ws_ab, ws_bc, transforms, dims = args
-        p = _stage_plan(A, transforms, tuple(dims), Pencil(), False)
-        return p.execute(A, C, stream=stream)
+        return _run_stages(A, C, transforms, tuple(dims), Pencil(), False, stream)
     if len(args) == 6 and isinstance(args[5], Slab):
         B, A, ws, transforms, dims, decomp = args
-        p = _stage_plan(A, transforms, tuple(dims), decomp, False)
-        return p.execute(A, B, stream=stream)
+        return _run_stages(A, B, transforms, tuple(dims), decomp, False, stream)
     raise TypeError("fft_: expected (C, A, B, ws_AB, ws_BC, transforms, dims) or (B, A, ws, transforms, dims, Slab())")
 
 
 def ifft_(*args, stream=None):
-    """ifft!(C, A, B, ws_AB, ws_BC, transforms, dims) [Pencil, input A, output C]
-    or ifft!(A, B, ws, transforms, dims, Slab()) [input B, output A]; chunked
-    arrays as in fft_."""
+    """ifft!(C, A, B, ws_AB, ws_BC, transforms, dims) [Pencil, input A, output
+    C: transforms[3] along dims[3], then [2], then [1]] or ifft!(A, B, ws,
+    transforms, dims, Slab()) [input B, output A] (src/fft.jl:691-724,
+    753-776); chunked arrays as in fft_."""
     if args and isinstance(args[0], chunked.DArray):
         return chunked.ifft_(*args, stream=stream)
     if len(args) == 7:
         C, A, B, ws_ab, ws_bc, transforms, dims = args
-        p = _stage_plan(A, transforms, tuple(dims), Pencil(), True)
-        return p.execute(A, C, stream=stream)
+        return _run_stages(A, C, transforms, tuple(dims), Pencil(), True, stream)
     if len(args) == 6 and isinstance(args[5], Slab):
         A, B, ws, transforms, dims, decomp = args
-        p = _stage_plan(B, transforms, tuple(dims), decomp, True)
-        return p.execute(B, A, stream=stream)
+        return _run_stages(B, A, transforms, tuple(dims), decomp, True, stream)
     raise TypeError("ifft_: expected (C, A, B, ws_AB, ws_BC, transforms, dims) or (A, B, ws, transforms, dims, Slab())")
 
 

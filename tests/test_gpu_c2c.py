@@ -138,3 +138,50 @@ def test_slab_plane_pipeline_matches_pencil(gpu, shape):
     p32 = D.Plan(shape, L.C2C, -1, L.F32, D.Slab(), (1, 2, 3), flags=L.FLAG_PLANE_PIPELINE)
     F32 = p32.execute(_dev(a32), D.DeviceArray(shape, np.complex64))
     assert O.rel_l2(F32.to_host().astype(np.complex128), O.fftn(a32.astype(np.complex128))) <= TOL32
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("shape,planes", [((64, 48, 40), 1), ((30, 14, 9), 3), ((32, 24, 20), 7)])
+def test_slab_plane_pipeline_many_chunks(gpu, monkeypatch, shape, planes, prec):
+    """The plane pipeline with chunks of a few z-planes (DFFT_SLAB_CHUNK_BYTES
+    set to `planes` planes of the first pass's output), so every chunked pass
+    runs at nonzero plane offsets: C2C forward and inverse, R2C and C2R,
+    against the oracle."""
+    import daggerfft as D
+    from daggerfft import _lib as L
+    P = L.F64 if prec == "f64" else L.F32
+    cdt = np.complex128 if prec == "f64" else np.complex64
+    rdt = np.float64 if prec == "f64" else np.float32
+    tol = TOL64 if prec == "f64" else TOL32
+    es = 16 if prec == "f64" else 8
+    nh = shape[0] // 2 + 1
+    monkeypatch.setenv("DFFT_SLAB_CHUNK_BYTES", str(planes * shape[0] * shape[1] * es))
+
+    def chk(y, ref):
+        e = O.rel_l2(np.asarray(y, dtype=np.complex128 if np.iscomplexobj(y) else np.float64), ref)
+        print(shape, planes, prec, "rel-L2", e)
+        assert e <= tol
+
+    a = O.fill(31, shape, cdt)
+    a64 = a.astype(np.complex128)
+    pf = D.Plan(shape, L.C2C, -1, P, D.Slab(), (1, 2, 3), flags=L.FLAG_PLANE_PIPELINE)
+    pi = D.Plan(shape, L.C2C, +1, P, D.Slab(), (1, 2, 3), flags=L.FLAG_PLANE_PIPELINE)
+    assert pf.schedule()[0] == "plane_pipeline" and pi.schedule()[0] == "plane_pipeline"
+    assert pf.schedule()[1] > 4 and pi.schedule()[1] > 4  # several chunks
+    F = pf.execute(_dev(a), D.DeviceArray(shape, cdt))
+    chk(F.to_host(), O.fftn(a64))
+    B = pi.execute(_dev(a), D.DeviceArray(shape, cdt))
+    chk(B.to_host(), O.fftn(a64, inverse=True))
+
+    r = O.fill(32, shape, rdt)
+    r64 = r.astype(np.float64)
+    pr = D.Plan(shape, L.R2C, -1, P, D.Slab(), (1, 2, 3), flags=L.FLAG_PLANE_PIPELINE)
+    assert pr.schedule()[0] == "plane_pipeline"
+    Fr = pr.execute(_dev(r), D.DeviceArray((nh,) + shape[1:], cdt))
+    ref = O.rfftn(r64)
+    chk(Fr.to_host(), ref)
+    pc = D.Plan(shape, L.C2R, +1, P, D.Slab(), (1, 2, 3), flags=L.FLAG_PLANE_PIPELINE)
+    assert pc.schedule()[0] == "plane_pipeline"
+    spec = np.asfortranarray(ref.astype(cdt))
+    Br = pc.execute(_dev(spec), D.DeviceArray(shape, rdt))
+    chk(Br.to_host(), O.irfftn(spec.astype(np.complex128), shape[0]))

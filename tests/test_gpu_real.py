@@ -47,20 +47,41 @@ def test_rfft_first_dim_strided(gpu, dims):
     assert O.rel_l2(B.to_host(), a) <= TOL64
 
 
-def test_rfft_256_cubed(gpu):
+_R256 = {}
+
+
+def _ref256():
+    """Seed-2 Float64 256^3 input and its oracle rfft, computed once."""
+    if not _R256:
+        a = O.fill(2, (256, 256, 256), np.float64)
+        _R256["a"] = a
+        _R256["F"] = O.rfftn(a)
+    return _R256["a"], _R256["F"]
+
+
+@pytest.mark.parametrize("decomp", ["pencil", "slab"])
+def test_rfft_256_cubed(gpu, decomp):
+    """The 256^3 Float64 R2C/C2R configuration, both decompositions: every
+    output element against the oracle (rfft), and irfft against the oracle's
+    irfft of the same spectrum and against the input."""
     import daggerfft as D
+    dec = D.Pencil() if decomp == "pencil" else D.Slab()
     shape = (256, 256, 256)
+    a, ref = _ref256()
     A = D.rand(shape, np.float64, seed=2)
-    a = O.fill(2, shape, np.float64)
-    F = D.rfft(A)
-    ref = O.rfftn(a)
+    F = D.rfft(A, decomp=dec)
     e = O.rel_l2(F.to_host(), ref)
-    print("256^3 rfft rel-L2", e)
+    print("256^3", decomp, "rfft rel-L2", e)
     assert e <= TOL64
-    B = D.irfft(F, 256)
+    B = D.irfft(F, 256, decomp=dec)
     e2 = O.rel_l2(B.to_host(), a)
-    print("256^3 irfft round trip rel-L2", e2)
+    print("256^3", decomp, "irfft round trip rel-L2", e2)
     assert e2 <= TOL64
+    # C2R alone, from the oracle's spectrum
+    B2 = D.irfft(D.DeviceArray.from_host(ref), 256, decomp=dec)
+    e3 = O.rel_l2(B2.to_host(), O.irfftn(ref, 256))
+    print("256^3", decomp, "irfft rel-L2 vs oracle irfft", e3)
+    assert e3 <= TOL64
 
 
 def test_irfft_ignores_dc_nyquist_imag(gpu):

@@ -62,3 +62,29 @@ def test_concurrent_plans_and_streams(gpu):
         th.join(120)
     assert not any(th.is_alive() for th in threads)
     assert not errors, errors[:5]
+
+
+def test_irfft_cached_plan_on_two_streams(gpu):
+    """irfft through the process-wide plan cache from two streams at once:
+    each call gets its own C2R workspace, so the two results equal the
+    oracle's (the plan's own workspace would be shared by both)."""
+    import torch
+    import daggerfft as D
+    shape = (96, 80, 64)
+    refs, specs = [], []
+    for k in range(2):
+        a = O.fill(60 + k, shape, np.float64)
+        spec = O.rfftn(a)
+        refs.append(O.irfftn(spec, shape[0]))
+        specs.append(D.DeviceArray.from_host(spec))
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    for rep in range(5):
+        outs = []
+        for k in range(2):
+            outs.append(D.irfft(specs[k], shape[0], stream=streams[k]))
+        for s in streams:
+            s.synchronize()
+        for k in range(2):
+            e = O.rel_l2(outs[k].to_host(), refs[k])
+            assert e <= 1e-12, (rep, k, e)

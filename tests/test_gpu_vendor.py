@@ -1,7 +1,8 @@
 """Cross-check against the platform's FFT library (rocFFT through torch.fft,
-hipFFT) at the configuration sizes, including the full 512^3 ComplexF64 block
-that the host oracle is too slow for: 3D C2C forward and inverse, R2C and
-C2R, both decompositions, both precisions.  The library under test runs
+hipFFT) at the configuration sizes, including the full 512^3 ComplexF64
+block: 3D C2C forward and inverse, R2C and C2R, both decompositions, both
+precisions.  This is a second, independent implementation beside the oracle
+(tests/test_gpu_fullsize.py compares the same 512^3 block with the oracle).  The library under test runs
 through its C-ABI; torch.fft is only the comparison (an independent
 implementation, so agreement pins sign, normalisation, axis order and the
 half-spectrum layout at full size).
