@@ -94,14 +94,53 @@ def vendor_fft(A, X, n, reps=5):
             "roundtrip_rel_l2": rt}
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(n):
-    """The oracle's restatement of the reference pipeline (per-axis batched
-    1D FFTs in stage order) on the host cores, one full n^3 forward + inverse."""
+    """The reference's CPU path (Julia + Dagger + FFTW) cannot run on this
+    host (no Julia, no FFTW).  Its stand-in is the fastest host FFT library
+    present, scipy.fft (pocketfft, the same algorithm family as FFTW), on the
+    same cores: `value`.  Beside it, the oracle's restatement of the
+    reference pipeline (per-axis batched 1D FFTs in the reference's stage
+    order, oracle/dfft_oracle.c), one full n^3 forward + inverse.  The sample
+    is bounded: five pocketfft forward + inverse pairs and one oracle pair of
+    the same 512^3 ComplexF64 block (about 10 s of host time)."""
     import numpy as np
 
     from oracle import oracle as O
     shape = (n, n, n)
+    N = float(n) ** 3
+    flop = 5 * N * math.log2(N)
+    cores = O.num_threads()  # the host cores this process may use (the box's CPU share)
     a = O.fill(3, shape, np.complex128)
+    out = {"unit": "GFLOP/s", "cores": cores, "nproc": os.cpu_count(), "cpu_model": _cpu_model()}
+    try:
+        import scipy.fft as sf
+        x = np.ascontiguousarray(a.T)  # the same column-major block as a row-major (n3, n2, n1) array
+        times = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            y = sf.fftn(x, workers=cores)
+            t1 = time.perf_counter()
+            sf.ifftn(y, workers=cores, overwrite_x=True)
+            t2 = time.perf_counter()
+            times.append(((t1 - t0) + (t2 - t1)) / 2)
+        ts = float(np.median(times))
+        del x, y
+        out.update({"value": round(flop * 1e-9 / ts, 3), "kind": "port",
+                    "impl": "scipy.fft.fftn/ifftn (pocketfft C++), workers=%d" % cores,
+                    "sample": "five forward + inverse pairs of the %d^3 ComplexF64 block; median %.1f ms per "
+                              "transform" % (n, ts * 1e3)})
+    except Exception as e:  # scipy absent: the oracle restatement becomes the value
+        out["library_error"] = repr(e)
     t0 = time.perf_counter()
     for ax in range(3):
         O.c2c_axis(a, ax, -1, 1.0)
@@ -109,27 +148,15 @@ def cpu_baseline(n):
     for ax in (2, 1, 0):
         O.c2c_axis(a, ax, +1, 1.0 / n)
     t2 = time.perf_counter()
-    N = float(n) ** 3
     t = ((t1 - t0) + (t2 - t1)) / 2
-    out = {"value": round(5 * N * math.log2(N) * 1e-9 / t, 3), "unit": "GFLOP/s", "cores": O.num_threads(),
-           "kind": "port", "sample": "one full %d^3 ComplexF64 forward + inverse through the C oracle "
-           "(per-axis batched FFTs in the reference's stage order), time per transform %.3f s" % (n, t)}
-    # a library FFT on the same host cores beside it (pocketfft, the algorithm
-    # family of the reference's FFTW CPU path; FFTW itself is not installed)
-    try:
-        import scipy.fft as sf
-        workers = O.num_threads()  # the same cores as the oracle (the box's CPU share)
-        x = a.astype(np.complex128, copy=True)
-        t0 = time.perf_counter()
-        y = sf.fftn(x, workers=workers)
-        t1 = time.perf_counter()
-        sf.ifftn(y, workers=workers, overwrite_x=True)
-        t2 = time.perf_counter()
-        ts = ((t1 - t0) + (t2 - t1)) / 2
-        out["scipy_pocketfft"] = {"value": round(5 * N * math.log2(N) * 1e-9 / ts, 3), "unit": "GFLOP/s",
-                                  "workers": workers, "ms_per_transform": round(ts * 1e3, 1)}
-    except Exception as e:  # scipy absent: the oracle port stays the baseline
-        out["scipy_pocketfft"] = {"error": repr(e)}
+    restatement = {"value": round(flop * 1e-9 / t, 3), "unit": "GFLOP/s", "threads": cores,
+                   "sample": "one full %d^3 ComplexF64 forward + inverse through the C oracle (textbook "
+                             "recursive mixed-radix FFT per axis, OpenMP over lines), %.3f s per transform"
+                             % (n, t)}
+    if "value" not in out:
+        out.update({"value": restatement["value"], "kind": "port", "impl": "the C oracle",
+                    "sample": restatement["sample"]})
+    out["restatement"] = restatement
     return out
 
 
@@ -366,6 +393,10 @@ def serving(count):
     out["sequential_cold"] = {k: round(v, 4) for k, v in cold.items()}
     out["throughput_vs_sequential_warm"] = round(sat["req_per_s"] / warm["req_per_s"], 3)
     out["throughput_vs_sequential_cold"] = round(sat["req_per_s"] / cold["req_per_s"], 3)
+    out["throughput_claim"] = ("the >=3x target is met against cold sequential calls (each call builds its plan "
+                               "and allocates its arrays, the reference's per-call life cycle); against warm "
+                               "calls (plans cached, buffers pooled) the served stream gains only the per-call "
+                               "host and synchronisation gaps")
     worst = {"f32": 0.0, "f64": 0.0}
     same = 0
     for r in keep:
@@ -555,16 +586,16 @@ def main():
         results["slab"] = {"ms_step": ms_step, "fwd_ms": fwd, "inv_ms": inv, "err": roundtrip_err(),
                            "schedule": sched, "launches": launches, "bytes": sf.algorithmic_bytes()}
 
-    # measured device-to-device copy ceiling on the same buffers
-    torch.cuda.synchronize()
-    c0, c1 = Ev(), Ev()
-    X.tensor.copy_(A.tensor)
-    c0.record()
-    for _ in range(5):
-        X.tensor.copy_(A.tensor)
-    c1.record()
-    torch.cuda.synchronize()
-    copy_gbps = 2 * A.nbytes * 5 / (c0.elapsed_time(c1) / 1e3) / 1e9
+    # measured device-to-device copy ceilings on the same 2 GiB buffers: the
+    # library's streaming 16 B/lane copy kernel, and torch's copy
+    import ctypes
+    stream = D._stream_handle(None)
+
+    def dcopy():
+        L.check(L.lib().dfft_device_copy(ctypes.c_void_p(X.ptr), ctypes.c_void_p(A.ptr), A.nbytes, stream))
+
+    copy_gbps = 2 * A.nbytes / (_time_events(dcopy, 10) / 1e3) / 1e9
+    torch_copy_gbps = 2 * A.nbytes / (_time_events(lambda: X.tensor.copy_(A.tensor), 5) / 1e3) / 1e9
 
     vendor = None
     if not args.no_vendor:
@@ -594,7 +625,8 @@ def main():
            "dims": [1, 2, 3], "step": "fft then ifft", "time_per_transform_ms": round(ms_step / 2, 4),
            "hbm_frac_per_transform": round(r["bytes"] / t_transform_s / 1e9 / HBM_PEAK_GBPS, 4),
            "algorithmic_bytes_per_transform": r["bytes"], "roundtrip_rel_l2": r["err"],
-           "d2d_copy_GBps": round(copy_gbps, 1)}
+           "copy_ceiling_GBps": round(copy_gbps, 1), "torch_copy_GBps": round(torch_copy_gbps, 1),
+           "hbm_frac_of_copy_ceiling": round(r["bytes"] / t_transform_s / 1e9 / copy_gbps, 4)}
     for k, v in results.items():
         cfg["%s_ms_per_transform" % k] = round(v["ms_step"] / 2, 4)
         cfg["%s_GFLOPs" % k] = round(5 * N * math.log2(N) * 1e-9 / (v["ms_step"] / 2 / 1e3), 1)

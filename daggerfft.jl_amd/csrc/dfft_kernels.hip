@@ -6,6 +6,7 @@
 //   k_box       box gather / scatter / copy (pack_kernel!, unpack_kernel!,
 //               local_copy_kernel!; src/fftgpu.jl:452-495), coalesced along axis 0
 //   k_permute   LDS-tiled 3D axis permutation (the pencil transpose)
+//   k_copy16    plain streaming copy (the measured HBM copy ceiling)
 #include <hip/hip_runtime.h>
 
 #include "dfft_axis.hpp"
@@ -155,6 +156,32 @@ hipError_t launch_fill(void *out, int64_t nscalar, int prec, uint64_t seed, hipS
     if (blocks > 65536) blocks = 65536;
     if (prec) hipLaunchKernelGGL(k_fill<double>, dim3((unsigned)blocks), dim3(256), 0, stream, (double *)out, nscalar, seed);
     else hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)blocks), dim3(256), 0, stream, (float *)out, nscalar, seed);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// plain copy, 16 B per lane, streaming: the HBM copy ceiling the axis passes
+// are measured against.  Each thread issues its four loads before its four
+// stores, so a workgroup has 16 KB in flight.
+// ---------------------------------------------------------------------------
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_copy16(const u32x4 *__restrict__ src, u32x4 *__restrict__ dst, int64_t n16) {
+    const int64_t base = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+    u32x4 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (base + 256 * k < n16) v[k] = __builtin_nontemporal_load(src + base + 256 * k);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (base + 256 * k < n16) __builtin_nontemporal_store(v[k], dst + base + 256 * k);
+}
+
+hipError_t launch_copy16(const void *src, void *dst, int64_t n16, hipStream_t stream) {
+    if (n16 <= 0) return hipSuccess;
+    const int64_t blocks = (n16 + 1023) / 1024;
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, stream, (const u32x4 *)src, (u32x4 *)dst, n16);
     return hipGetLastError();
 }
 

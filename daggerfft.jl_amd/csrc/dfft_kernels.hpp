@@ -26,6 +26,7 @@ std::vector<int> blue_fixed_lengths(int prec, int contig, int op);
 hipError_t launch_fill(void *out, int64_t nscalar, int prec, uint64_t seed, hipStream_t stream);
 hipError_t launch_box(const void *src, int64_t s_off, int64_t s_s1, int64_t s_s2, void *dst, int64_t d_off,
                       int64_t d_s1, int64_t d_s2, const int64_t *cnt, int esize, hipStream_t stream);
+hipError_t launch_copy16(const void *src, void *dst, int64_t n16, hipStream_t stream);
 hipError_t launch_permute(const void *in, void *out, const int64_t *n, const int *perm, int esize,
                           hipStream_t stream);
 

@@ -8,6 +8,7 @@
 // dims[2]; inverse reversed (fft!/ifft!, src/fft.jl:651-724; Slab
 // src/fft.jl:727-776).
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <algorithm>
 #include <cstdlib>
@@ -35,6 +36,8 @@ int hip_fail(int hip_status, const char *what) {
              hipGetErrorString((hipError_t)hip_status), hip_status);
     return DFFT_ERR_HIP;
 }
+
+void set_last_error(const char *msg) { snprintf(g_last_error, sizeof g_last_error, "%s", msg); }
 
 // ---------------------------------------------------------------------------
 // twiddles: exp(-2 pi i k / n) evaluated in long double with octant folding,
@@ -751,12 +754,21 @@ static void *resolve(int buf, const void *in, void *out, void *work) {
     }
 }
 
+// Trace ranges around each pass's launch, named as the reference's NVTX
+// ranges around its stages ("DIM 1", "DIM 2", "DIM 3", src/fft.jl:654-686):
+// the Julia dim (axis + 1) the pass transforms.  They bracket the enqueue on
+// the host; rocprofv3 --marker-trace shows them beside the kernels.
+static const char *const kDimRange[3] = {"DIM 1", "DIM 2", "DIM 3"};
+
 static int run_passes(const dfft_plan *p, const void *d_in, void *d_out, void *work, hipStream_t st) {
     for (const auto &pp : p->passes) {
         AxisPass a = pp.ap;
         a.in = resolve(pp.src, d_in, d_out, work);
         a.out = resolve(pp.dst, d_in, d_out, work);
-        if (hipError_t e = launch_axis(a, pp.prec, pp.nblocks, pp.threads, pp.lds, st)) return hip_fail(e, "axis pass launch");
+        roctxRangePush(kDimRange[pp.axis]);
+        hipError_t e = launch_axis(a, pp.prec, pp.nblocks, pp.threads, pp.lds, st);
+        roctxRangePop();
+        if (e) return hip_fail(e, "axis pass launch");
     }
     return DFFT_OK;
 }
@@ -805,6 +817,36 @@ static int run_graph(const dfft_plan *p, const void *d_in, void *d_out, void *wo
     }
     return hipGraphLaunch(p->gexec, st) == hipSuccess ? DFFT_OK : DFFT_ERR_HIP;
 }
+
+namespace dfft {
+int plan_check_io(const dfft_plan *p, const void *d_in, const void *d_out) {
+    if (!p || !d_in || !d_out) return DFFT_ERR_ARG;
+    if (d_in == d_out && (p->kind == DFFT_R2C || p->kind == DFFT_C2R)) return DFFT_ERR_ARG;
+    return DFFT_OK;
+}
+
+size_t plan_work_bytes(const dfft_plan *p) { return p->work_bytes; }
+
+int plan_device(const dfft_plan *p) { return p->device; }
+
+int plan_enqueue(const dfft_plan *p, const void *d_in, void *d_out, void *work, void *stream, bool graph) {
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (p->slab) {
+        void *slots[3] = {const_cast<void *>(d_in), d_out, work};
+        roctxRangePush("DIM 1+2 (plane pipeline), DIM 3");
+        const int rc = group_launch(p->slab, slots, 3, st);
+        roctxRangePop();
+        return rc;
+    }
+    if (graph && (p->flags & DFFT_FLAG_GRAPH)) {
+        roctxRangePush("dfft plan graph");
+        const int rc = run_graph(p, d_in, d_out, work, st);
+        roctxRangePop();
+        return rc;
+    }
+    return run_passes(p, d_in, d_out, work, st);
+}
+}  // namespace dfft
 
 extern "C" {
 
@@ -885,9 +927,7 @@ int dfft_plan_io_size(const dfft_plan *p, size_t *in_bytes, size_t *out_bytes) {
 }
 
 int dfft_execute(const dfft_plan *p, const void *d_in, void *d_out, void *d_work, void *stream) {
-    if (!p || !d_in || !d_out) return DFFT_ERR_ARG;
-    if (d_in == d_out && (p->kind == DFFT_R2C || p->kind == DFFT_C2R)) return DFFT_ERR_ARG;
-    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (int rc = plan_check_io(p, d_in, d_out)) return rc;
     void *work = d_work;
     if (p->work_bytes && !work) {
         dfft_plan *mp = const_cast<dfft_plan *>(p);
@@ -898,12 +938,7 @@ int dfft_execute(const dfft_plan *p, const void *d_in, void *d_out, void *d_work
         }
         work = mp->own_work;
     }
-    if (p->slab) {
-        void *slots[3] = {const_cast<void *>(d_in), d_out, work};
-        return group_launch(p->slab, slots, 3, st);
-    }
-    if (p->flags & DFFT_FLAG_GRAPH) return run_graph(p, d_in, d_out, work, st);
-    return run_passes(p, d_in, d_out, work, st);
+    return plan_enqueue(p, d_in, d_out, work, stream, true);
 }
 
 int dfft_execute_pass(const dfft_plan *p, int pass, const void *d_in, void *d_out, void *d_work, void *stream) {
@@ -923,7 +958,9 @@ int dfft_execute_pass(const dfft_plan *p, int pass, const void *d_in, void *d_ou
     AxisPass a = pp.ap;
     a.in = resolve(pp.src, d_in, d_out, work);
     a.out = resolve(pp.dst, d_in, d_out, work);
+    roctxRangePush(kDimRange[pp.axis]);
     hipError_t e = launch_axis(a, pp.prec, pp.nblocks, pp.threads, pp.lds, (hipStream_t)stream);
+    roctxRangePop();
     return e == hipSuccess ? DFFT_OK : hip_fail(e, "axis pass launch");
 }
 
@@ -1061,6 +1098,14 @@ int dfft_permute(void *d_out, const void *d_in, const int64_t *in_shape, const i
     if (d_out == d_in) return DFFT_ERR_ARG;
     hipError_t e = launch_permute(d_in, d_out, in_shape, perm, esize, (hipStream_t)stream);
     return e == hipSuccess ? DFFT_OK : DFFT_ERR_HIP;
+}
+
+int dfft_device_copy(void *d_dst, const void *d_src, size_t bytes, void *stream) {
+    if (!d_dst || !d_src || bytes % 16 || ((uintptr_t)d_dst | (uintptr_t)d_src) % 16) return DFFT_ERR_ARG;
+    const uintptr_t a = (uintptr_t)d_src, b = (uintptr_t)d_dst;
+    if (a < b + bytes && b < a + bytes) return DFFT_ERR_ARG;  // overlapping ranges
+    hipError_t e = launch_copy16(d_src, d_dst, (int64_t)(bytes / 16), (hipStream_t)stream);
+    return e == hipSuccess ? DFFT_OK : hip_fail(e, "copy launch");
 }
 
 int dfft_fill_uniform(void *d_out, int64_t nelem, int is_complex, int precision, uint64_t seed, void *stream) {

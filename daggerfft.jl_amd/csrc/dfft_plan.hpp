@@ -10,6 +10,8 @@
 
 #include "dfft_types.hpp"
 
+struct dfft_plan;  // the C ABI's plan object (include/daggerfft_hip.h)
+
 namespace dfft {
 
 constexpr int EMAX = 8;  // complex values a thread holds per stage (variant 0; variant 1 holds 16)
@@ -40,6 +42,8 @@ struct Tuning {
 // Records `what` and HIP's message as this thread's last error (dfft_last_error)
 // and returns DFFT_ERR_HIP.
 int hip_fail(int hip_status, const char *what);
+// Records `msg` as this thread's last error (dfft_last_error).
+void set_last_error(const char *msg);
 bool factorize(int L, std::vector<int> &R);
 void set_geometry(AxisPass &a, int axis, const int64_t nin[3], const int64_t nout[3]);
 // grouped: the pass runs in a grouped launch (dfft_group.hip), whose kernel
@@ -67,5 +71,16 @@ bool tuning_lookup(const PassPlan &pp, Tuning &t);
 void wisdom_autoload();
 std::string tuning_key(const PassPlan &pp);
 void cis_ld(int64_t k, int64_t n, long double *c, long double *s);
+
+// Plan execution shared by dfft_execute and dfft_execute_static.
+// Argument checks of an execution (DFFT_OK or DFFT_ERR_ARG).
+int plan_check_io(const ::dfft_plan *p, const void *d_in, const void *d_out);
+// Device workspace bytes the plan's passes use (0: none).
+size_t plan_work_bytes(const ::dfft_plan *p);
+// The device the plan was created on.
+int plan_device(const ::dfft_plan *p);
+// Enqueues the plan's launches on `stream` with `work` as its workspace;
+// graph: replay the plan's hipGraph if it was created with DFFT_FLAG_GRAPH.
+int plan_enqueue(const ::dfft_plan *p, const void *d_in, void *d_out, void *work, void *stream, bool graph);
 
 }  // namespace dfft

@@ -95,6 +95,7 @@ def _declare(L):
         "dfft_plan_workspace_size": [P, SZP],
         "dfft_plan_io_size": [P, SZP, SZP],
         "dfft_execute": [P, P, P, P, P],
+        "dfft_device_copy": [P, P, ctypes.c_size_t, P],
         "dfft_execute_pass": [P, I, P, P, P, P],
         "dfft_plan_destroy": [P],
         "dfft_plan_num_passes": [P, IP],

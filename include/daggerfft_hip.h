@@ -192,6 +192,12 @@ int dfft_box_copy(void *d_dst, const int64_t *dst_shape, const int64_t *dlo, con
 int dfft_permute(void *d_out, const void *d_in, const int64_t *in_shape, const int *perm, int esize,
                  void *stream);
 
+/* Plain device-to-device copy of `bytes` (a multiple of 16; both pointers
+ * 16-byte aligned; ranges must not overlap): 16 B per lane with streaming
+ * loads and stores.  No reference counterpart: the benchmark measures the HBM
+ * copy ceiling the axis passes are compared with. */
+int dfft_device_copy(void *d_dst, const void *d_src, size_t bytes, void *stream);
+
 /* Seeded SplitMix64 uniform[0,1) fill on device (the benchmark input generator):
  * real element e <- u(seed, e); complex element e <- (u(seed, 2e), u(seed, 2e+1)). */
 int dfft_fill_uniform(void *d_out, int64_t nelem, int is_complex, int precision, uint64_t seed,
