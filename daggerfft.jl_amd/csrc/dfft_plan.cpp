@@ -1040,6 +1040,7 @@ const char *dfft_strerror(int s) {
     case DFFT_ERR_STATE: return "object used in the wrong state";
     case DFFT_ERR_NOT_READY: return "request not finished";
     case DFFT_ERR_UNSUPPORTED: return "transform kind not implemented on the GPU path";
+    case DFFT_ERR_SCRATCH: return "the plan's workspace exceeds the static scratch buffer (DFFT_STATIC_SCRATCH_BYTES)";
     }
     return "unknown status";
 }

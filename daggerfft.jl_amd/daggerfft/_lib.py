@@ -22,6 +22,8 @@ DFFT_ERR_BUDGET = 5
 DFFT_ERR_STATE = 6
 DFFT_ERR_NOT_READY = 7
 DFFT_ERR_UNSUPPORTED = 8
+DFFT_ERR_SCRATCH = 9
+STATIC_SCRATCH_BYTES = 256 << 20
 
 C2C, R2C, C2R, R2R = 0, 1, 2, 3
 F32, F64 = 0, 1
@@ -95,6 +97,7 @@ def _declare(L):
         "dfft_plan_workspace_size": [P, SZP],
         "dfft_plan_io_size": [P, SZP, SZP],
         "dfft_execute": [P, P, P, P, P],
+        "dfft_execute_static": [P, P, P, P],
         "dfft_device_copy": [P, P, ctypes.c_size_t, P],
         "dfft_execute_pass": [P, I, P, P, P, P],
         "dfft_plan_destroy": [P],

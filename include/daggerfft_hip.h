@@ -55,6 +55,7 @@ extern "C" {
 #define DFFT_ERR_STATE             6  /* object used in the wrong state */
 #define DFFT_ERR_NOT_READY         7  /* serving: request not finished yet */
 #define DFFT_ERR_UNSUPPORTED       8  /* transform kind not implemented on the GPU path */
+#define DFFT_ERR_SCRATCH           9  /* dfft_execute_static: the plan's workspace exceeds DFFT_STATIC_SCRATCH_BYTES */
 
 /* ---- transform kinds (FFT / RFFT / IRFFT tags, src/fft.jl:17-24; R2R README.md:59-66) */
 #define DFFT_C2C  0
@@ -134,6 +135,27 @@ int dfft_plan_io_size(const dfft_plan *plan, size_t *in_bytes, size_t *out_bytes
  */
 int dfft_execute(const dfft_plan *plan, const void *d_in, void *d_out, void *d_work, void *stream);
 
+/*
+ * dfft_execute without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * caller passes the input and output arrays; a plan that needs a workspace
+ * (C2R without DFFT_FLAG_DESTROY_INPUT) uses the library's static scratch
+ * buffer of DFFT_STATIC_SCRATCH_BYTES (256 MiB) on the current device, which
+ * each device holds from its first such call until the process ends.
+ *   - DFFT_ERR_SCRATCH if the plan's workspace is larger (nothing is launched;
+ *     dfft_last_error gives both sizes);
+ *   - DFFT_ERR_ARG if the current device is not the plan's device;
+ *   - every call on one device shares that device's scratch buffer and it
+ *     keeps the last call's contents (the plan's first pass overwrites the
+ *     part it uses before any pass reads it), so calls that overlap on one
+ *     device -- on different streams, or from different threads -- race;
+ *   - plans created with DFFT_FLAG_GRAPH run their passes as plain launches
+ *     here (instantiating the graph would allocate).
+ */
+#define DFFT_STATIC_SCRATCH_BYTES ((size_t)256 << 20)
+int dfft_execute_static(const dfft_plan *plan, const void *d_in, void *d_out, void *stream);
+
 int dfft_plan_destroy(dfft_plan *plan);
 
 /* Execute one axis pass of the plan (0 <= pass < num_passes), so a caller can
@@ -166,7 +188,8 @@ int dfft_plan_schedule(const dfft_plan *plan, int *schedule, int *launches);
 int dfft_plan_algorithmic_bytes(const dfft_plan *plan, double *bytes);
 
 const char *dfft_strerror(int status);
-/* The calling thread's last DFFT_ERR_HIP cause: the failing HIP call and HIP's message. */
+/* The calling thread's last DFFT_ERR_HIP cause (the failing HIP call and HIP's
+ * message) or DFFT_ERR_SCRATCH cause (the two sizes). */
 const char *dfft_last_error(void);
 int dfft_version(int *major, int *minor);
 
