@@ -82,6 +82,14 @@ function execute!(p::Plan, out::ROCArray, inp::ROCArray)
 end
 LinearAlgebra_mul!(out, p::Plan, inp) = execute!(p, out, inp)
 
+# execute! without any device allocation (for code under graph capture): a C2R
+# plan's workspace is the library's static per-device scratch buffer
+function execute_static!(p::Plan, out::ROCArray, inp::ROCArray)
+    check(ccall((:dfft_execute_static, libdfft), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                p.handle, pointer(inp), pointer(out), stream_ptr()))
+    return out
+end
+
 _dims(A, dims) = dims === nothing ? Tuple(1:ndims(A)) : Tuple(dims)
 
 function r2r_kinds(transforms, dims, decomp)
