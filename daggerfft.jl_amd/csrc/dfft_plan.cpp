@@ -897,6 +897,13 @@ int dfft_plan_create(dfft_plan **plan, int rank, const int64_t *n, int kind, int
         return hip_fail(e, "hipGetDevice");
     }
     if (!(flags & DFFT_FLAG_NO_TUNE)) wisdom_autoload();
+    if ((flags & DFFT_FLAG_MEASURE) && !(flags & DFFT_FLAG_NO_TUNE)) {
+        const int mrc = measure_plan(rank, n, kind, sign, r2r_kinds, precision, decomp, dims, ndims, flags);
+        if (mrc != DFFT_OK) {
+            delete p;
+            return mrc;
+        }
+    }
     int rc;
     try {
         rc = build_plan(p);
@@ -962,6 +969,28 @@ int dfft_execute_pass(const dfft_plan *p, int pass, const void *d_in, void *d_ou
     hipError_t e = launch_axis(a, pp.prec, pp.nblocks, pp.threads, pp.lds, (hipStream_t)stream);
     roctxRangePop();
     return e == hipSuccess ? DFFT_OK : hip_fail(e, "axis pass launch");
+}
+
+int dfft_plan_export_wisdom(const dfft_plan *p, const char *path) {
+    if (!p || !path) return DFFT_ERR_ARG;
+    std::vector<std::pair<std::string, Tuning>> entries;
+    for (const auto &pp : p->passes) {
+        const AxisPass &a = pp.ap;
+        Tuning t;
+        // a Bluestein pass keeps only its engine (variant 0: runtime, else the
+        // compile-time default); every other pass its radix list and tile shape
+        if (!a.blue)
+            for (int s = 0; s < a.sp.S; ++s) t.radices.push_back(a.sp.R[s]);
+        t.tpl = a.variant ? 0 : a.TPL;
+        t.tc = a.TC;
+        t.pad_shift = a.contig ? a.pad_shift : 0;
+        t.variant = a.variant;
+        t.swz = a.swz;
+        const std::string key = tuning_key(pp);
+        if (std::none_of(entries.begin(), entries.end(), [&](const auto &e) { return e.first == key; }))
+            entries.emplace_back(key, t);
+    }
+    return wisdom_write(path, entries);
 }
 
 int dfft_plan_destroy(dfft_plan *p) {
@@ -1041,6 +1070,7 @@ const char *dfft_strerror(int s) {
     case DFFT_ERR_NOT_READY: return "request not finished";
     case DFFT_ERR_UNSUPPORTED: return "transform kind not implemented on the GPU path";
     case DFFT_ERR_SCRATCH: return "the plan's workspace exceeds the static scratch buffer (DFFT_STATIC_SCRATCH_BYTES)";
+    case DFFT_ERR_WISDOM: return "wisdom file stamped for another HIP build or GPU architecture";
     }
     return "unknown status";
 }

@@ -6,6 +6,7 @@
 
 #include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "dfft_types.hpp"
@@ -68,6 +69,11 @@ struct TwiddleTable {
     int upload(void **d_ptr, size_t *bytes) const;
 };
 bool tuning_lookup(const PassPlan &pp, Tuning &t);
+// Writes `entries` as a stamped wisdom file.
+int wisdom_write(const char *path, const std::vector<std::pair<std::string, Tuning>> &entries);
+// DFFT_FLAG_MEASURE: tune the passes of this plan that have no cache entry.
+int measure_plan(int rank, const int64_t *n, int kind, int sign, const int *r2r_kinds, int precision, int decomp,
+                 const int *dims, int ndims, uint32_t flags);
 void wisdom_autoload();
 std::string tuning_key(const PassPlan &pp);
 void cis_ld(int64_t k, int64_t n, long double *c, long double *s);

@@ -35,6 +35,68 @@ static std::mutex g_mu;
 static std::unordered_map<std::string, Tuning> g_cache;
 static std::once_flag g_wisdom_once;
 static bool g_wisdom_done = false;  // guarded by g_mu
+static std::string g_autosave;      // guarded by g_mu: where tuning results are saved ("" none)
+
+// The stamp of a wisdom file: the HIP version the library's kernels were
+// compiled with and the GPU architecture its schedules were measured on.
+// The compiler decides the kernels' code and so which schedule is fastest;
+// the runtime in a process may be another build (PyTorch ships its own), and
+// does not change the kernels.  A stamped file of another build or
+// architecture is refused (FFTW likewise ignores wisdom of another FFTW
+// build); hand-written files without a stamp are schedule overrides and
+// always load.
+static int build_version() { return HIP_VERSION; }
+
+static std::string device_arch() {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return "unknown";
+    }
+    std::string arch = prop.gcnArchName;
+    return arch.substr(0, arch.find(':'));
+}
+
+static std::string stamp_line() {
+    return "# daggerfft_hip schedule cache hip=" + std::to_string(build_version()) + " arch=" + device_arch() +
+           ": key radices tpl tc pad_shift variant swizzle\n";
+}
+
+static void write_entry(std::ostream &f, const std::string &key, const Tuning &t) {
+    f << key << " ";
+    for (size_t i = 0; i < t.radices.size(); ++i) f << (i ? "," : "") << t.radices[i];
+    if (t.radices.empty()) f << "-";
+    f << " " << t.tpl << " " << t.tc << " " << t.pad_shift << " " << t.variant << " " << t.swz << "\n";
+}
+
+// Writes `entries` under the stamp (the caller holds no lock).
+int wisdom_write(const char *path, const std::vector<std::pair<std::string, Tuning>> &entries) {
+    if (!path || !*path) return DFFT_ERR_ARG;
+    std::ofstream f(path);
+    if (!f) return DFFT_ERR_ARG;
+    f << stamp_line();
+    for (const auto &kv : entries) write_entry(f, kv.first, kv.second);
+    return f.good() ? DFFT_OK : DFFT_ERR_ARG;
+}
+
+static std::vector<std::pair<std::string, Tuning>> snapshot() {
+    std::lock_guard<std::mutex> g(g_mu);
+    std::vector<std::pair<std::string, Tuning>> v(g_cache.begin(), g_cache.end());
+    std::sort(v.begin(), v.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+    return v;
+}
+
+// save_wisdom() after the cache changed (src/fft.jl:122-124,162), when a
+// save path is set (dfft_tune_autosave or $DFFT_WISDOM_SAVE)
+static void autosave() {
+    std::string path;
+    {
+        std::lock_guard<std::mutex> g(g_mu);
+        path = g_autosave;
+    }
+    if (!path.empty()) (void)wisdom_write(path.c_str(), snapshot());
+}
 
 static int size_class(const AxisPass &a) {
     const double lines = (double)a.nlines * (double)a.nouter;
@@ -49,7 +111,7 @@ std::string tuning_key(const PassPlan &pp) {
            std::to_string(layout) + "/" + std::to_string(size_class(a));
 }
 
-static int load_file(const char *path);
+static int load_file(const char *path, bool quiet = false);
 
 // The schedule cache shipped beside the library (the reference imports its
 // FFTW wisdom file at module init, src/fft.jl:116-120): $DFFT_WISDOM if set
@@ -62,6 +124,10 @@ void wisdom_autoload() {
             if (g_wisdom_done) return;
             g_wisdom_done = true;
         }
+        if (const char *e = getenv("DFFT_WISDOM_SAVE")) {
+            std::lock_guard<std::mutex> g(g_mu);
+            g_autosave = e;
+        }
         std::string path;
         if (const char *e = getenv("DFFT_WISDOM")) {
             if (!strcmp(e, "off") || !*e) return;
@@ -72,14 +138,11 @@ void wisdom_autoload() {
             std::string lib = info.dli_fname;
             const size_t slash = lib.rfind('/');
             const std::string dir = slash == std::string::npos ? "." : lib.substr(0, slash);
-            int dev = 0;
-            hipDeviceProp_t prop;
-            if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return;
-            std::string arch = prop.gcnArchName;
-            arch = arch.substr(0, arch.find(':'));
+            const std::string arch = device_arch();
+            if (arch == "unknown") return;
             path = dir + "/wisdom/" + arch + ".txt";
         }
-        (void)load_file(path.c_str());
+        (void)load_file(path.c_str(), true);
     });
 }
 
@@ -205,10 +268,35 @@ PassPlan plan_pass(const dfft_plan *p, int i);
 size_t plan_buffer_bytes(const dfft_plan *p, int buf);
 }  // namespace dfft
 
+namespace dfft {
+static int autotune_impl(int rank, const int64_t *n, int kind, int sign, const int *r2r_kinds, int precision,
+                         int decomp, const int *dims, int ndims, uint32_t flags, int level, void *stream,
+                         bool only_missing);
+
+// DFFT_FLAG_MEASURE at plan creation: the passes that have no schedule yet
+// are timed first (FFTW.MEASURE, src/fft.jl:127), and the cache is saved if a
+// save path is set.
+int measure_plan(int rank, const int64_t *n, int kind, int sign, const int *r2r_kinds, int precision, int decomp,
+                 const int *dims, int ndims, uint32_t flags) {
+    return autotune_impl(rank, n, kind, sign, r2r_kinds, precision, decomp, dims, ndims,
+                         flags & ~(uint32_t)DFFT_FLAG_MEASURE, 1, nullptr, true);
+}
+}  // namespace dfft
+
 extern "C" {
 
 int dfft_autotune(int rank, const int64_t *n, int kind, int sign, const int *r2r_kinds, int precision, int decomp,
                   const int *dims, int ndims, uint32_t flags, int level, void *stream) {
+    return autotune_impl(rank, n, kind, sign, r2r_kinds, precision, decomp, dims, ndims,
+                         flags & ~(uint32_t)DFFT_FLAG_MEASURE, level, stream, false);
+}
+
+}  // extern "C"
+
+namespace dfft {
+static int autotune_impl(int rank, const int64_t *n, int kind, int sign, const int *r2r_kinds, int precision,
+                         int decomp, const int *dims, int ndims, uint32_t flags, int level, void *stream,
+                         bool only_missing) {
     if (level < 0) return DFFT_ERR_ARG;
     if (level == 0) return DFFT_OK;
     dfft_plan *p = nullptr;
@@ -263,6 +351,8 @@ int dfft_autotune(int rank, const int64_t *n, int kind, int sign, const int *r2r
         PassPlan base = plan_pass(p, i);
         const std::string key = tuning_key(base);
         if (!done.insert(key).second) continue;
+        Tuning have;
+        if (only_missing && tuning_lookup(base, have)) continue;
         // first a quick timing of every candidate, then the best few again
         // with more launches, keeping the best of three trials: candidates
         // within a few per cent of each other are otherwise ranked by noise
@@ -308,24 +398,19 @@ int dfft_autotune(int rank, const int64_t *n, int kind, int sign, const int *r2r
     for (void *b : buf)
         if (b) (void)hipFree(b);
     dfft_plan_destroy(p);
+    if (rc == DFFT_OK) autosave();
     return rc;
 }
+}  // namespace dfft
 
-int dfft_tune_save(const char *path) {
-    if (!path) return DFFT_ERR_ARG;
-    std::ofstream f(path);
-    if (!f) return DFFT_ERR_ARG;
+extern "C" {
+
+int dfft_tune_save(const char *path) { return wisdom_write(path, snapshot()); }
+
+int dfft_tune_autosave(const char *path) {
     std::lock_guard<std::mutex> g(g_mu);
-    f << "# daggerfft_hip schedule cache: key radices tpl tc pad_shift variant swizzle\n";
-    for (const auto &kv : g_cache) {
-        f << kv.first << " ";
-        for (size_t i = 0; i < kv.second.radices.size(); ++i) f << (i ? "," : "") << kv.second.radices[i];
-        if (kv.second.radices.empty()) f << "-";
-        f << " " << kv.second.tpl << " " << kv.second.tc << " " << kv.second.pad_shift << " " << kv.second.variant
-          << " " << kv.second.swz
-          << "\n";
-    }
-    return f.good() ? DFFT_OK : DFFT_ERR_ARG;
+    g_autosave = path ? path : "";
+    return DFFT_OK;
 }
 
 int dfft_tune_load(const char *path) {
@@ -336,12 +421,34 @@ int dfft_tune_load(const char *path) {
 }  // extern "C"
 
 namespace dfft {
-static int load_file(const char *path) {
+static int load_file(const char *path, bool quiet) {
     std::ifstream f(path);
     if (!f) return DFFT_ERR_ARG;
     std::string line;
+    std::vector<std::pair<std::string, Tuning>> entries;
     while (std::getline(f, line)) {
-        if (line.empty() || line[0] == '#') continue;
+        if (line.empty()) continue;
+        if (line[0] == '#') {
+            // the stamp: hip=<HIP version of the build> arch=<gfx arch>
+            std::istringstream hs(line);
+            std::string tok;
+            while (hs >> tok) {
+                if (!tok.empty() && tok.back() == ':') tok.pop_back();
+                const bool hip = tok.rfind("hip=", 0) == 0, arch = tok.rfind("arch=", 0) == 0;
+                if (!hip && !arch) continue;
+                const std::string val = tok.substr(hip ? 4 : 5);
+                const std::string cur = hip ? std::to_string(build_version()) : device_arch();
+                if (val != cur && val != "unknown" && cur != "unknown" && cur != "0") {
+                    if (!quiet) {
+                        const std::string msg = std::string("wisdom file ") + path + " has " + tok +
+                                                ", this process runs " + (hip ? "hip=" : "arch=") + cur;
+                        set_last_error(msg.c_str());
+                    }
+                    return DFFT_ERR_WISDOM;
+                }
+            }
+            continue;
+        }
         std::istringstream ss(line);
         std::string key, rad;
         Tuning t;
@@ -363,8 +470,10 @@ static int load_file(const char *path) {
         }
         if (t.tpl < 0 || t.tpl > 1024 || t.tc < 0 || t.tc > 1024 || t.pad_shift < 0 || t.pad_shift > 30)
             return DFFT_ERR_ARG;
-        tuning_store(key, t);
+        entries.emplace_back(key, t);
     }
+    // all or nothing: a file that fails to parse leaves the cache as it was
+    for (const auto &kv : entries) tuning_store(kv.first, kv.second);
     return DFFT_OK;
 }
 

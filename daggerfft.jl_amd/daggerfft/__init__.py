@@ -279,6 +279,12 @@ class Plan:
         check(lib().dfft_plan_schedule(self._h, ctypes.byref(k), ctypes.byref(n)))
         return ("plane_pipeline" if k.value == 1 else "passes"), n.value
 
+    def export_wisdom(self, path):
+        """Write this plan's pass schedules as a stamped wisdom file: a process
+        that loads it (tune_load) and creates the same plan gets the same
+        kernels and launch geometry."""
+        check(lib().dfft_plan_export_wisdom(self._h, str(path).encode()), "dfft_plan_export_wisdom")
+
     def algorithmic_bytes(self):
         b = ctypes.c_double()
         check(lib().dfft_plan_algorithmic_bytes(self._h, ctypes.byref(b)))
@@ -348,7 +354,15 @@ def autotune(shape, kind=L.C2C, sign=-1, precision=L.F64, decomp=Pencil(), dims=
 
 
 def tune_save(path):
+    """Write the schedule cache as a wisdom file stamped with the HIP runtime
+    version and GPU architecture (export_wisdom, src/fft.jl:122-124)."""
     check(lib().dfft_tune_save(str(path).encode()), "dfft_tune_save")
+
+
+def tune_autosave(path):
+    """Save the cache to `path` whenever tuning adds to it (None: off), as the
+    reference saves its wisdom after every new plan (src/fft.jl:162)."""
+    check(lib().dfft_tune_autosave(str(path).encode() if path else None), "dfft_tune_autosave")
 
 
 def tune_load(path):

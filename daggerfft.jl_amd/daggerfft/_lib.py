@@ -23,6 +23,7 @@ DFFT_ERR_STATE = 6
 DFFT_ERR_NOT_READY = 7
 DFFT_ERR_UNSUPPORTED = 8
 DFFT_ERR_SCRATCH = 9
+DFFT_ERR_WISDOM = 10
 STATIC_SCRATCH_BYTES = 256 << 20
 
 C2C, R2C, C2R, R2R = 0, 1, 2, 3
@@ -37,6 +38,7 @@ FLAG_STAGE_ORDER = 0x20
 FLAG_PLANE_PIPELINE = 0x40
 FLAG_BATCH_GROUPED = 0x80
 FLAG_BATCH_LANES = 0x100
+FLAG_MEASURE = 0x200
 
 _lib = None
 _lock = threading.Lock()
@@ -46,7 +48,7 @@ class DFFTError(RuntimeError):
     def __init__(self, status, where=""):
         self.status = status
         msg = strerror(status) if _lib is not None else str(status)
-        if _lib is not None and status == DFFT_ERR_HIP:
+        if _lib is not None and status in (DFFT_ERR_HIP, DFFT_ERR_SCRATCH, DFFT_ERR_WISDOM):
             msg += " [" + _lib.dfft_last_error().decode() + "]"
         super().__init__("%s: %s (status %d)" % (where or "libdaggerfft_hip", msg, status))
 
@@ -118,6 +120,8 @@ def _declare(L):
         "dfft_batch_destroy": [P],
         "dfft_autotune": [I, I64P, I, I, IP, I, I, IP, I, ctypes.c_uint32, I, P],
         "dfft_tune_save": [ctypes.c_char_p],
+        "dfft_tune_autosave": [ctypes.c_char_p],
+        "dfft_plan_export_wisdom": [P, ctypes.c_char_p],
         "dfft_tune_load": [ctypes.c_char_p],
         "dfft_tune_clear": [],
         "dfft_tune_entries": [IP],

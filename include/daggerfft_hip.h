@@ -56,6 +56,7 @@ extern "C" {
 #define DFFT_ERR_NOT_READY         7  /* serving: request not finished yet */
 #define DFFT_ERR_UNSUPPORTED       8  /* transform kind not implemented on the GPU path */
 #define DFFT_ERR_SCRATCH           9  /* dfft_execute_static: the plan's workspace exceeds DFFT_STATIC_SCRATCH_BYTES */
+#define DFFT_ERR_WISDOM           10  /* a wisdom file stamped for another HIP build or GPU architecture */
 
 /* ---- transform kinds (FFT / RFFT / IRFFT tags, src/fft.jl:17-24; R2R README.md:59-66) */
 #define DFFT_C2C  0
@@ -97,6 +98,9 @@ extern "C" {
                                           (grouped launches re-reading each chunk from the Infinity Cache) */
 #define DFFT_FLAG_BATCH_GROUPED  0x80u /* batch: one grouped launch per axis-pass level (three launches) */
 #define DFFT_FLAG_BATCH_LANES    0x100u /* batch: each transform's own plan on four forked streams */
+#define DFFT_FLAG_MEASURE        0x200u /* time the schedules of passes the cache lacks at plan
+                                         * creation (FFTW.MEASURE, src/fft.jl:127); synchronises
+                                         * the device and saves the cache if dfft_tune_autosave is set */
 
 typedef struct dfft_plan dfft_plan;
 
@@ -236,10 +240,27 @@ int dfft_fill_uniform(void *d_out, int64_t nelem, int is_complex, int precision,
  * allocates scratch buffers of the plan's size and synchronises `stream`. */
 int dfft_autotune(int rank, const int64_t *n, int kind, int sign, const int *r2r_kinds, int precision,
                   int decomp, const int *dims, int ndims, uint32_t flags, int level, void *stream);
+/* Wisdom files start with a stamp naming the HIP version the library was
+ * compiled with and the GPU architecture (hip=... arch=...).  dfft_tune_load
+ * refuses a stamped file of another build or architecture with DFFT_ERR_WISDOM (dfft_last_error says
+ * which); unstamped hand-written files load as schedule overrides.  The
+ * library imports <library dir>/wisdom/<arch>.txt (or $DFFT_WISDOM) at its
+ * first plan creation, as the reference imports its wisdom at module init
+ * (src/fft.jl:116-120). */
 int dfft_tune_save(const char *path);
 int dfft_tune_load(const char *path);
 int dfft_tune_clear(void);
 int dfft_tune_entries(int *count);
+/* Save the whole cache to `path` whenever tuning adds to it (dfft_autotune,
+ * DFFT_FLAG_MEASURE plans): the reference's save_wisdom() after every new plan
+ * (src/fft.jl:122-124,162).  NULL or "" turns it off; $DFFT_WISDOM_SAVE sets it
+ * at the first plan creation. */
+int dfft_tune_autosave(const char *path);
+/* Plan serialisation: write the schedules of this plan's passes (the tuned
+ * ones or the built-in choice) as a stamped wisdom file.  A process that
+ * loads it and creates a plan with the same arguments gets the same kernels,
+ * radix lists, tile shapes and launch geometry. */
+int dfft_plan_export_wisdom(const dfft_plan *plan, const char *path);
 
 /* ---- batched launch ------------------------------------------------------
  * No reference counterpart: many independent transforms per kernel launch
