@@ -327,7 +327,8 @@ def _forward_table(cases):
         ms = _time_events(lambda: pf.execute(A, X), 10)
         out["%s_%s" % ("x".join(map(str, sh)), "f64" if prec else "f32")] = {
             "ms": round(ms, 4), "GBps": round(pf.algorithmic_bytes() / ms / 1e6, 1), "roundtrip_rel_l2": rt,
-            "passes": [{"axis": q["axis"], "L": q["L"], "M": q["bluestein"], "kernel": q["kernel"].split("<")[0]}
+            "passes": [{"axis": q["axis"], "L": q["L"], "M": q["bluestein"] or q["rader"],
+                        "kernel": q["kernel"].split("<")[0]}
                        for q in pf.passes()]}
         del A, X, Y
     torch.cuda.empty_cache()

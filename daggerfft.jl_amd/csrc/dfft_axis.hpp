@@ -423,7 +423,8 @@ constexpr int stage_tpl(const int *R, int S, int L, int emax) {
     }
     return t;
 }
-constexpr void min_stage_search(int rem, int start, Factors &cur, int L, int emax, Factors &best, int &best_tpl) {
+constexpr void min_stage_search(int rem, int start, Factors &cur, int L, int emax, Factors &best, int &best_tpl,
+                                int rmax = 16) {
     constexpr int set[13] = {16, 15, 14, 12, 10, 9, 8, 7, 6, 5, 4, 3, 2};
     if (rem == 1) {
         const int t = stage_tpl(cur.R, cur.S, L, emax);
@@ -436,11 +437,18 @@ constexpr void min_stage_search(int rem, int start, Factors &cur, int L, int ema
     if (cur.S >= MAX_STAGES || (best.S >= 0 && cur.S + 1 > best.S)) return;
     for (int i = start; i < 13; ++i) {
         const int r = set[i];
-        if (r > emax || rem % r) continue;
+        if (r > emax || r > rmax || rem % r) continue;
         cur.R[cur.S++] = r;
-        min_stage_search(rem / r, i, cur, L, emax, best, best_tpl);
+        min_stage_search(rem / r, i, cur, L, emax, best, best_tpl, rmax);
         --cur.S;
     }
+}
+// the fewest stages with radices up to rmax, 16 values per thread
+constexpr Factors factors_capped(int L, int rmax) {
+    Factors cur{0, {0}}, best{-1, {0}};
+    int best_tpl = 0;
+    min_stage_search(L, 0, cur, L, 16, best, best_tpl, rmax);
+    return best;
 }
 constexpr Factors factors_of(int L, int emax) {
     Factors f{0, {0}};
@@ -468,6 +476,16 @@ struct RadicesOfImpl<L, EMAX, std::integer_sequence<int, I...>> {
 };
 template <int L, int EMAX>
 using RadicesOf = typename RadicesOfImpl<L, EMAX, std::make_integer_sequence<int, factors_of(L, EMAX).S>>::type;
+template <int L, int RMAX, class Seq>
+struct RadicesCapImpl;
+template <int L, int RMAX, int... I>
+struct RadicesCapImpl<L, RMAX, std::integer_sequence<int, I...>> {
+    using type = Radices<factors_capped(L, RMAX).R[I]...>;
+};
+// fewest stages with radices up to 10 (16 values per thread): the FP64
+// two-FFT tiles spill with the larger composite butterflies
+template <int L>
+using RadicesCap10 = typename RadicesCapImpl<L, 10, std::make_integer_sequence<int, factors_capped(L, 10).S>>::type;
 template <int L>
 using RadicesOf8 = RadicesOf<L, 8>;  // (single template argument: usable inside macros)
 template <int L>
@@ -809,6 +827,66 @@ __device__ __forceinline__ void blue_fixed_tile(const AxisPass &a, const void *i
     fixed_stages_from<RR, EMAX, TPLC, SPLIT, 0>(c1, v);
     st_store_global<RR::R[RR::S - 1], EMAX, M, TPLC>(c1, v);
     if constexpr (OP == OP_R2C) r2c_post_lds(c, TPLC);
+}
+
+// ---------------------------------------------------------------------------
+// Rader tiles, for prime line lengths n whose n-1 is 7-smooth (Rader 1968).
+// With a primitive root g mod n, j = g^q and k = g^-p turn the DFT into
+//   X_{g^-p} = x_0 + sum_{q<n-1} x_{g^q} w_n^{g^(q-p)} = x_0 + (a (*) b)_p,
+// a cyclic convolution of length M = n-1 (no zero padding) of
+// a_q = x_{g^q} and b_m = w_n^{g^-m}, and X_0 = x_0 + sum_q a_q = x_0 + A_0.
+// The two M-point FFTs run as in the compile-time Bluestein tile (the
+// spectrum product in registers between them); the chirp block holds the
+// index pairs (g^p mod n, g^-p mod n) and bhat holds FFT_M(b) / M.  M is
+// about n instead of 2n-1 or more, so a Rader pass does 2-2.5x less FFT work
+// than the Bluestein pass of the same n.  C2C only.
+// ---------------------------------------------------------------------------
+template <class T, int OP, bool CONTIG, int PHASE>
+struct RaderCtx : TileCtx<T, OP, CONTIG> {
+    typedef TileCtx<T, OP, CONTIG> Op;  // the op's maps; a.sp.L = n
+    const cx<T> *perm, *bhat;           // perm[p] = (g^p mod n, g^-p mod n) as values of T
+    cx<T> x0;                           // the line's x_0 as the op loads it
+
+    __device__ __forceinline__ RaderCtx(const Op &c, cx<T> x0_)
+        : Op(c), perm(c.twb() + c.a.chirp), bhat(c.twb() + c.a.bhat), x0(x0_) {}
+    __device__ __forceinline__ int len() const { return this->a.bsp.L; }
+    __device__ __forceinline__ int stage_tw(int s) const {
+        return PHASE == 0 ? this->a.bsp.tw[s] : this->a.bsp2.tw[s];
+    }
+    __device__ __forceinline__ cx<T> load0(int p) const {  // phase 0: a_p = x_{g^p}
+        return Op::load0((int)perm[p].x);
+    }
+    __device__ __forceinline__ void store_last(int p, cx<T> v) const {  // phase 1: X_{g^-p}
+        Op::store_last((int)perm[p].y, x0 + conj(v));
+    }
+};
+
+template <class RL, class T, int OP, bool CONTIG, bool SPLIT>
+__device__ __forceinline__ void rader_fixed_tile(const AxisPass &a, const void *in, void *out, int64_t tile, int tid,
+                                                 cx<T> *lds) {
+    static_assert(OP == OP_C2C, "Rader passes are C2C");
+    constexpr int EMAX = 16, TPLC = fixed_tpl<RL, EMAX>(), M = RL::L;
+    using RR = Reversed<RL>;
+    int ls, t;
+    bool valid;
+    int64_t ib, ob;
+    tile_map<CONTIG>(a, tile, tid, TPLC, ls, t, valid, ib, ob);
+    TileCtx<T, OP, CONTIG> c{a, in, out, lds, ls, t, valid, ib, ob};
+    const cx<T> x0 = t < (1 << 28) ? c.load0(0) : cx<T>{0, 0};
+    const RaderCtx<T, OP, CONTIG, 0> c0(c, x0);
+    const RaderCtx<T, OP, CONTIG, 1> c1(c, x0);
+    cx<T> v[EMAX];
+    st_load_global<RL::R[0], EMAX, M, TPLC>(c0, v);
+    st_compute<RL::R[0], EMAX, M, TPLC, 1>(c0, v, 0);
+    fixed_stages_from<RL, EMAX, TPLC, SPLIT, 0>(c0, v);
+    // position 0 of the first FFT's output (A_0 = sum of a) sits in v[0] of
+    // the line's thread 0
+    const cx<T> a0 = v[0];
+    blue_turn<RL::R[RL::S - 1], EMAX, M, TPLC>(c0, v);
+    st_compute<RR::R[0], EMAX, M, TPLC, 1>(c1, v, 0);
+    fixed_stages_from<RR, EMAX, TPLC, SPLIT, 0>(c1, v);
+    st_store_global<RR::R[RR::S - 1], EMAX, M, TPLC>(c1, v);
+    if (t == 0) c.store_last(0, x0 + a0);
 }
 
 }  // namespace dfft

@@ -81,4 +81,29 @@ axis_kernel_t blue_fixed_kernel(int prec, int contig, int op, int M) {
     return e ? e->fn : nullptr;
 }
 
+// Rader kernels (dfft_rader_*.hip): C2C of prime lengths M + 1
+extern const FixedTable g_rader_a, g_rader_b, g_rader_c, g_rader_d;
+
+static const FixedEntry *rader_fixed_entry(int prec, int contig, int M) {
+    for (const FixedTable *t : {&g_rader_a, &g_rader_b, &g_rader_c, &g_rader_d})
+        for (int i = 0; i < t->n; ++i) {
+            const FixedEntry &e = t->e[i];
+            if (e.prec == (prec ? 1 : 0) && e.contig == (contig ? 1 : 0) && e.L == M) return &e;
+        }
+    return nullptr;
+}
+
+bool rader_fixed_schedule(int prec, int contig, int M, std::vector<int> &radices, int &tpl) {
+    const FixedEntry *e = rader_fixed_entry(prec, contig, M);
+    if (!e) return false;
+    radices.assign(e->R, e->R + e->S);
+    tpl = e->tpl;
+    return true;
+}
+
+axis_kernel_t rader_fixed_kernel(int prec, int contig, int M) {
+    const FixedEntry *e = rader_fixed_entry(prec, contig, M);
+    return e ? e->fn : nullptr;
+}
+
 }  // namespace dfft

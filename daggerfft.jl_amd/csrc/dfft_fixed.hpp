@@ -45,6 +45,8 @@ struct FixedTable {
 // op); compile-time C2C engine of (precision, layout, convolution length)
 axis_kernel_t blue_kernel(int prec, int contig, int op);
 axis_kernel_t blue_fixed_kernel(int prec, int contig, int op, int M);
+// the Rader kernels (dfft_rader_*.hip): C2C of prime length M + 1
+axis_kernel_t rader_fixed_kernel(int prec, int contig, int M);
 
 #define DFFT_FX(T, P, OPV, CONTIG, VAR, EMAXV, SPLIT, RL)                                             \
     {P, OPV, CONTIG, VAR, RL::L, fixed_tpl<RL, EMAXV>(), RL::S, RL::R,                                 \

@@ -95,6 +95,7 @@ std::vector<std::vector<int>> fixed_lists(int prec, int contig, int variant, int
 }
 
 static axis_kernel_t axis_kernel(const AxisPass &a, int prec) {
+    if (a.blue == 2) return rader_fixed_kernel(prec, a.contig, a.bsp.L);
     if (a.blue) return a.variant ? blue_fixed_kernel(prec, a.contig, a.op, a.bsp.L) : blue_kernel(prec, a.contig, a.op);
     if (a.variant >= 1) {
         const FixedEntry *e = fixed_entry(prec, a.contig, a.variant, a.op, a.sp.L, a.sp.R, a.sp.S);

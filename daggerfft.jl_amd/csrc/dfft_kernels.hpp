@@ -23,6 +23,8 @@ std::vector<std::vector<int>> fixed_lists(int prec, int contig, int variant, int
 bool blue_fixed_schedule(int prec, int contig, int op, int M, std::vector<int> &radices, int &tpl, int &variant);
 // the convolution lengths that have such a schedule
 std::vector<int> blue_fixed_lengths(int prec, int contig, int op);
+// compile-time Rader schedule (C2C, split exchange) of the prime length M + 1, if built
+bool rader_fixed_schedule(int prec, int contig, int M, std::vector<int> &radices, int &tpl);
 hipError_t launch_fill(void *out, int64_t nscalar, int prec, uint64_t seed, hipStream_t stream);
 hipError_t launch_box(const void *src, int64_t s_off, int64_t s_s1, int64_t s_s2, void *dst, int64_t d_off,
                       int64_t d_s1, int64_t d_s2, const int64_t *cnt, int esize, hipStream_t stream);

@@ -156,11 +156,95 @@ static void build_bluestein(AxisPass &ap, std::vector<long double> &tab) {
     }
 }
 
+static int64_t powmod(int64_t b, int64_t e, int64_t m) {
+    int64_t r = 1;
+    b %= m;
+    for (; e > 0; e >>= 1, b = b * b % m)
+        if (e & 1) r = r * b % m;
+    return r;
+}
+
+// The smallest primitive root of the prime n: g^((n-1)/q) != 1 for every
+// prime factor q of n - 1.
+static int primitive_root(int n) {
+    std::vector<int> qs;
+    int m = n - 1;
+    for (int q = 2; q * q <= m; ++q)
+        if (m % q == 0) {
+            qs.push_back(q);
+            while (m % q == 0) m /= q;
+        }
+    if (m > 1) qs.push_back(m);
+    for (int g = 2; g < n; ++g) {
+        bool ok = true;
+        for (int q : qs)
+            if (powmod(g, (n - 1) / q, n) == 1) {
+                ok = false;
+                break;
+            }
+        if (ok) return g;
+    }
+    return 1;  // n = 2
+}
+
+static bool is_prime(int n) {
+    if (n < 2) return false;
+    for (int q = 2; q * q <= n; ++q)
+        if (n % q == 0) return false;
+    return true;
+}
+
+// Rader blocks of a prime length-n pass convolving over M = n - 1 (Rader
+// 1968): the stage twiddles of M (both radix orders), the index pairs
+// (g^p mod n, g^-p mod n), p < M, stored as the two parts of one table entry
+// (exact in either precision for n < 2^24), and bhat = FFT_M(b) / M for
+// b_m = w_n^(g^-m), summed directly in long double.
+static void build_rader(AxisPass &ap, std::vector<long double> &tab) {
+    const int n = ap.sp.L, M = ap.bsp.L;
+    build_twiddles(ap.bsp, OP_C2C, M, tab);
+    build_twiddles(ap.bsp2, OP_C2C, M, tab);
+    const int g = primitive_root(n);
+    std::vector<int> gp(M), gi(M);
+    int64_t x = 1;
+    for (int p = 0; p < M; ++p) {
+        gp[p] = (int)x;
+        x = x * g % n;
+    }
+    for (int p = 0; p < M; ++p) gi[p] = gp[(M - p) % M];
+    ap.chirp = (int)(tab.size() / 2);
+    for (int p = 0; p < M; ++p) {
+        tab.push_back((long double)gp[p]);
+        tab.push_back((long double)gi[p]);
+    }
+    std::vector<long double> br(M), bi(M), cM(M), sM(M);
+    for (int m = 0; m < M; ++m) {
+        long double c, sn;
+        cis_ld(gi[m], n, &c, &sn);  // b_m = c - i sn
+        br[m] = c;
+        bi[m] = -sn;
+    }
+    for (int k = 0; k < M; ++k) cis_ld(k, M, &cM[k], &sM[k]);
+    ap.bhat = (int)(tab.size() / 2);
+    for (int k = 0; k < M; ++k) {
+        long double re = 0, im = 0;
+        int64_t mk = 0;  // m*k mod M
+        for (int m = 0; m < M; ++m) {
+            // b_m exp(-2 pi i m k / M)
+            re += br[m] * cM[mk] + bi[m] * sM[mk];
+            im += bi[m] * cM[mk] - br[m] * sM[mk];
+            mk += k;
+            if (mk >= M) mk -= M;
+        }
+        tab.push_back(re / M);
+        tab.push_back(im / M);
+    }
+}
+
 void TwiddleTable::add(AxisPass &ap) {
     std::string key = std::to_string(ap.op) + ":" + std::to_string(ap.sp.L);
     for (int s = 0; s < ap.sp.S; ++s) key += "," + std::to_string(ap.sp.R[s]);
     if (ap.blue) {
-        key += "/B" + std::to_string(ap.variant) + ":" + std::to_string(ap.bsp.L);
+        key += "/B" + std::to_string(ap.blue) + "." + std::to_string(ap.variant) + ":" + std::to_string(ap.bsp.L);
         for (int s = 0; s < ap.bsp.S; ++s) key += "," + std::to_string(ap.bsp.R[s]);
     }
     auto it = seen.find(key);
@@ -172,7 +256,8 @@ void TwiddleTable::add(AxisPass &ap) {
         ap.bhat = it->second.bhat;
     } else {
         build_twiddles(ap.sp, ap.op, ap.nlog, tab);
-        if (ap.blue) build_bluestein(ap, tab);
+        if (ap.blue == 2) build_rader(ap, tab);
+        else if (ap.blue) build_bluestein(ap, tab);
         seen[key] = TwiddleBlocks{ap.sp, ap.bsp, ap.bsp2, ap.chirp, ap.bhat};
     }
 }
@@ -240,12 +325,19 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
     int blue_fixed_tpl = 0, blue_fixed_variant = 0;
     {
         std::vector<int> f;
+        std::vector<int> rader_R;
+        int rader_tpl = 0;
         if (!factorize(a.sp.L, f)) {
             if (grouped) return DFFT_ERR_UNSUPPORTED_SIZE;
             const int m0 = 2 * a.sp.L - 1;
             const bool runtime = tune && tune->variant == 0;
             a.blue = 1;
             int Mf = 0;
+            // A prime C2C length whose n-1 has a compile-time Rader kernel may
+            // run Rader's convolution of length n-1 instead (decided below)
+            const bool rader_ok = !runtime && !(tune && tune->variant == 3) && a.op == OP_C2C &&
+                                  is_prime(a.sp.L) &&
+                                  rader_fixed_schedule(pp.prec, a.contig, a.sp.L - 1, rader_R, rader_tpl);
             if (!runtime) {
                 int64_t best = 0;
                 for (int m : blue_fixed_lengths(pp.prec, a.contig, a.op)) {
@@ -259,7 +351,26 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
                     }
                 }
             }
-            if (Mf) {
+            // Rader convolves over n-1 instead of M >= 2n-1, but every stage of
+            // its two FFTs is a barrier-separated LDS round trip like
+            // Bluestein's: it wins only when its FFT has no more stages
+            // (measured, FP64: 257 Rader over 8*8*4 1.7x faster than
+            // Bluestein over 16*8*5; 251 over 2*5*5*5 1.15x slower than over
+            // 16*16*2)
+            // (the schedule cache can name Rader: variant 2)
+            if (rader_ok) {
+                std::vector<int> br;
+                int bt = 0, bv = 0;
+                if ((tune && tune->variant == 2) || !Mf ||
+                    !blue_fixed_schedule(pp.prec, a.contig, a.op, Mf, br, bt, bv) || rader_R.size() <= br.size())
+                    a.blue = 2;
+            }
+            if (a.blue == 2) {
+                a.bsp.L = a.sp.L - 1;
+                blue_fixed_R = rader_R;
+                blue_fixed_tpl = rader_tpl;
+                blue_fixed_variant = 2;
+            } else if (Mf) {
                 (void)blue_fixed_schedule(pp.prec, a.contig, a.op, Mf, blue_fixed_R, blue_fixed_tpl,
                                           blue_fixed_variant);
                 a.bsp.L = Mf;
@@ -984,7 +1095,9 @@ int dfft_plan_export_wisdom(const dfft_plan *p, const char *path) {
         t.tpl = a.variant ? 0 : a.TPL;
         t.tc = a.TC;
         t.pad_shift = a.contig ? a.pad_shift : 0;
-        t.variant = a.variant;
+        // variant of a Bluestein pass: 0 runtime engine, 3 compile-time
+        // Bluestein, 2 Rader (or the compile-time default where no Rader kernel exists)
+        t.variant = a.blue == 1 ? (a.variant ? 3 : 0) : a.variant;
         t.swz = a.swz;
         const std::string key = tuning_key(pp);
         if (std::none_of(entries.begin(), entries.end(), [&](const auto &e) { return e.first == key; }))
@@ -1020,7 +1133,10 @@ int dfft_plan_describe(const dfft_plan *p, int pass, char *buf, size_t buflen) {
     // the kernel's name as profilers print it (spaces removed)
     const char *T = pp.prec ? "double" : "float";
     std::string kname;
-    if (a.blue && a.variant)
+    if (a.blue == 2)
+        kname = std::string("k_rader_fixed<") + T + "," + (a.contig ? "true" : "false") + ",true,dfft::Radices<" + rl +
+                ">>";
+    else if (a.blue && a.variant)
         kname = std::string("k_blue_fixed<") + T + "," + std::to_string(a.op) + "," + (a.contig ? "true" : "false") +
                 "," + (a.variant == 2 ? "true" : "false") + ",dfft::Radices<" + rl + ">>";
     else if (a.blue)
@@ -1034,10 +1150,10 @@ int dfft_plan_describe(const dfft_plan *p, int pass, char *buf, size_t buflen) {
     snprintf(buf, buflen,
              "{\"pass\":%d,\"axis\":%d,\"op\":\"%s\",\"L\":%d,\"nlog\":%d,\"radices\":\"%s\",\"TPL\":%d,\"TC\":%d,"
              "\"contig\":%d,\"variant\":%d,\"threads\":%d,\"blocks\":%lld,\"lds\":%zu,\"bytes\":%.0f,\"prec\":\"%s\","
-             "\"swz\":%d,\"bluestein\":%d,\"kernel\":\"%s\"}",
+             "\"swz\":%d,\"bluestein\":%d,\"rader\":%d,\"kernel\":\"%s\"}",
              pass, pp.axis, op_name(a.op), a.sp.L, a.nlog, rad.c_str(), a.TPL, a.TC, a.contig, a.variant, pp.threads,
-             (long long)pp.nblocks, pp.lds, pp.alg_bytes, pp.prec ? "f64" : "f32", a.swz, a.blue ? a.bsp.L : 0,
-             kname.c_str());
+             (long long)pp.nblocks, pp.lds, pp.alg_bytes, pp.prec ? "f64" : "f32", a.swz, a.blue == 1 ? a.bsp.L : 0,
+             a.blue == 2 ? a.bsp.L : 0, kname.c_str());
     return DFFT_OK;
 }
 

@@ -92,14 +92,15 @@ def test_r2r_kinds_bluestein(gpu, kind, shape):
     assert O.rel_l2(G.to_host(), ref) <= TOL64
 
 
-MIXED_M_EDGES = [257, 513, 769, 1025, 1537, 2049]
+# 263 and 773 are primes whose n-1 is not 7-smooth (so not Rader passes)
+MIXED_M_EDGES = [263, 513, 773, 1025, 1537, 2049]
 
 
 @pytest.mark.parametrize("n", MIXED_M_EDGES)
 @pytest.mark.parametrize("prec", [1, 0], ids=["f64", "f32"])
 def test_c2c_compile_time_convolution_lengths(gpu, n, prec):
     """Edges whose cheapest compile-time convolution length is a power of two
-    or 3*2^k / 5*2^k (257 -> 640, 513 -> 1280, 1025 -> 2560, ...), along
+    or 3*2^k / 5*2^k (263 -> 640, 513 -> 1280, 1025 -> 2560, ...), along
     each layout, against the oracle."""
     import daggerfft as D
     from daggerfft import _lib as L
@@ -122,12 +123,21 @@ def test_plan_describes_bluestein_passes(gpu):
     runs the runtime engine over a 7-smooth length."""
     import daggerfft as D
     from daggerfft import _lib as L
-    p = D.Plan((11, 16, 1009), L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3))
+    p = D.Plan((23, 16, 1013), L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3))
     by_axis = {q["axis"]: q for q in p.passes()}
-    assert by_axis[0]["bluestein"] == 32 and by_axis[0]["kernel"].startswith("k_blue_fixed<double,0,true,true")
-    assert D.Plan((257, 2, 2), L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3)).passes()[0]["bluestein"] == 640
+    assert by_axis[0]["bluestein"] == 48 and by_axis[0]["kernel"].startswith("k_blue_fixed<double,0,true,true")
+    assert D.Plan((263, 2, 2), L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3)).passes()[0]["bluestein"] == 640
     assert by_axis[1]["bluestein"] == 0
     assert by_axis[2]["bluestein"] == 2048 and by_axis[2]["kernel"].startswith("k_blue_fixed<double,0,false,true")
+    # primes whose n-1 is 7-smooth run Rader passes over n-1 where that FFT
+    # has no more stages than the Bluestein one: 11 (10, one stage against
+    # 16x2) and 257 (8x8x4 against 16x8x5) do; 1009 in FP64 (1008 needs four
+    # stages of radices up to 10 against 16x16x8) does not
+    p = D.Plan((11, 16, 1009), L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3))
+    by_axis = {q["axis"]: q for q in p.passes()}
+    assert by_axis[0]["rader"] == 10 and by_axis[0]["kernel"].startswith("k_rader_fixed<double,true")
+    assert by_axis[2]["bluestein"] == 2048 and by_axis[2]["kernel"].startswith("k_blue_fixed<double,0,false")
+    assert D.Plan((257, 2, 2), L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3)).passes()[0]["rader"] == 256
     r = D.Plan((22, 16, 8), L.R2C, -1, L.F64, D.Pencil(), (1, 2, 3))
     q = r.passes()[0]
     assert q["op"] == "R2C" and q["bluestein"] == 32 and q["kernel"].startswith("k_blue_fixed<double,1,true,false")
@@ -243,9 +253,10 @@ def test_lengths_beyond_one_tile_are_refused(gpu):
 
 
 def test_autotune_skips_bluestein_passes(gpu):
-    """The autotuner has no candidates for a Bluestein pass: tuning a shape
-    with prime axes records entries only for its smooth axes, and the plan
-    afterwards still runs Bluestein passes and matches the oracle."""
+    """The autotuner has no candidates for a convolution pass (Bluestein or
+    Rader): tuning a shape with prime axes records entries only for its
+    smooth axes, and the plan afterwards still runs convolution passes and
+    matches the oracle."""
     import daggerfft as D
     from daggerfft import _lib as L
     shape = (11, 64, 13)
@@ -254,7 +265,7 @@ def test_autotune_skips_bluestein_passes(gpu):
         D.autotune(shape, L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3), level=1)
         assert D.tune_entries() == 1  # the 64-point axis only
         p = D.Plan(shape, L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3))
-        assert [q["bluestein"] > 0 for q in p.passes() if q["axis"] != 1] == [True, True]
+        assert [q["bluestein"] > 0 or q["rader"] > 0 for q in p.passes() if q["axis"] != 1] == [True, True]
         a = O.fill(27, shape, np.complex128)
         F = p.execute(_dev(a), D.DeviceArray(shape, np.complex128))
         assert O.rel_l2(F.to_host(), O.fftn(a)) <= TOL64

@@ -46,8 +46,9 @@ def test_every_measured_pass_schedules(checker, wisdom):
 
 
 def test_bluestein_choices_on_the_host(tmp_path):
-    """Bluestein passes are chosen exactly for lengths with a prime factor
-    above 7, with a valid convolution length and schedule (tests/bluestein_check.cpp)."""
+    """Convolution passes are chosen exactly for lengths with a prime factor
+    above 7 -- Rader for C2C primes with 7-smooth n-1, Bluestein otherwise --
+    with a valid convolution length and schedule (tests/bluestein_check.cpp)."""
     if shutil.which("g++") is None or not os.path.exists(os.path.join(LIBDIR, "libdaggerfft_hip.so")):
         pytest.skip("needs g++ and the built library")
     exe = str(tmp_path / "bluestein_check")
