@@ -861,32 +861,77 @@ struct RaderCtx : TileCtx<T, OP, CONTIG> {
     }
 };
 
+// A contiguous line's permuted reads and writes, staged through the line's
+// LDS slot (complex values; the tile's LDS is sized for them): the line moves
+// between global memory and LDS in natural order, one coalesced burst, and
+// the permutations x_{g^q} / X_{g^-p} become LDS addressing.
+template <class T, int OP, int PHASE>
+struct RaderStagedCtx : RaderCtx<T, OP, true, PHASE> {
+    typedef RaderCtx<T, OP, true, PHASE> Base;
+    __device__ __forceinline__ RaderStagedCtx(const typename Base::Op &c, cx<T> x0_) : Base(c, x0_) {}
+    __device__ __forceinline__ cx<T> load0(int p) const { return this->lds[this->li((int)this->perm[p].x)]; }
+    __device__ __forceinline__ void store_last(int p, cx<T> v) const {
+        this->lds[this->li((int)this->perm[p].y)] = this->x0 + conj(v);
+    }
+};
+
 template <class RL, class T, int OP, bool CONTIG, bool SPLIT>
 __device__ __forceinline__ void rader_fixed_tile(const AxisPass &a, const void *in, void *out, int64_t tile, int tid,
                                                  cx<T> *lds) {
     static_assert(OP == OP_C2C, "Rader passes are C2C");
-    constexpr int EMAX = 16, TPLC = fixed_tpl<RL, EMAX>(), M = RL::L;
+    constexpr int EMAX = 16, TPLC = fixed_tpl<RL, EMAX>(), M = RL::L, N = M + 1;
     using RR = Reversed<RL>;
     int ls, t;
     bool valid;
     int64_t ib, ob;
     tile_map<CONTIG>(a, tile, tid, TPLC, ls, t, valid, ib, ob);
     TileCtx<T, OP, CONTIG> c{a, in, out, lds, ls, t, valid, ib, ob};
-    const cx<T> x0 = t < (1 << 28) ? c.load0(0) : cx<T>{0, 0};
-    const RaderCtx<T, OP, CONTIG, 0> c0(c, x0);
-    const RaderCtx<T, OP, CONTIG, 1> c1(c, x0);
     cx<T> v[EMAX];
-    st_load_global<RL::R[0], EMAX, M, TPLC>(c0, v);
-    st_compute<RL::R[0], EMAX, M, TPLC, 1>(c0, v, 0);
-    fixed_stages_from<RL, EMAX, TPLC, SPLIT, 0>(c0, v);
-    // position 0 of the first FFT's output (A_0 = sum of a) sits in v[0] of
-    // the line's thread 0
-    const cx<T> a0 = v[0];
-    blue_turn<RL::R[RL::S - 1], EMAX, M, TPLC>(c0, v);
-    st_compute<RR::R[0], EMAX, M, TPLC, 1>(c1, v, 0);
-    fixed_stages_from<RR, EMAX, TPLC, SPLIT, 0>(c1, v);
-    st_store_global<RR::R[RR::S - 1], EMAX, M, TPLC>(c1, v);
-    if (t == 0) c.store_last(0, x0 + a0);
+    if constexpr (CONTIG) {
+        // the whole line into its LDS slot in natural order
+        constexpr int NPT = (N + TPLC - 1) / TPLC;
+#pragma unroll
+        for (int k = 0; k < NPT; ++k) {
+            const int i = t + k * TPLC;
+            if (i < N) lds[c.li(i)] = c.load0(i);
+        }
+        __syncthreads();
+        const cx<T> x0 = t < (1 << 28) ? lds[c.li(0)] : cx<T>{0, 0};
+        const RaderStagedCtx<T, OP, 0> c0(c, x0);
+        const RaderStagedCtx<T, OP, 1> c1(c, x0);
+        st_load_global<RL::R[0], EMAX, M, TPLC>(c0, v);
+        __syncthreads();  // the first exchange reuses the slot
+        st_compute<RL::R[0], EMAX, M, TPLC, 1>(c0, v, 0);
+        fixed_stages_from<RL, EMAX, TPLC, SPLIT, 0>(c0, v);
+        const cx<T> a0 = v[0];
+        blue_turn<RL::R[RL::S - 1], EMAX, M, TPLC>(c0, v);
+        st_compute<RR::R[0], EMAX, M, TPLC, 1>(c1, v, 0);
+        fixed_stages_from<RR, EMAX, TPLC, SPLIT, 0>(c1, v);
+        // the last exchange ended with a barrier: X into the slot at g^-p, then out in natural order
+        st_store_global<RR::R[RR::S - 1], EMAX, M, TPLC>(c1, v);
+        if (t == 0) lds[c.li(0)] = x0 + a0;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NPT; ++k) {
+            const int i = t + k * TPLC;
+            if (i < N) c.store_last(i, lds[c.li(i)]);
+        }
+    } else {
+        const cx<T> x0 = t < (1 << 28) ? c.load0(0) : cx<T>{0, 0};
+        const RaderCtx<T, OP, CONTIG, 0> c0(c, x0);
+        const RaderCtx<T, OP, CONTIG, 1> c1(c, x0);
+        st_load_global<RL::R[0], EMAX, M, TPLC>(c0, v);
+        st_compute<RL::R[0], EMAX, M, TPLC, 1>(c0, v, 0);
+        fixed_stages_from<RL, EMAX, TPLC, SPLIT, 0>(c0, v);
+        // position 0 of the first FFT's output (A_0 = sum of a) sits in v[0]
+        // of the line's thread 0
+        const cx<T> a0 = v[0];
+        blue_turn<RL::R[RL::S - 1], EMAX, M, TPLC>(c0, v);
+        st_compute<RR::R[0], EMAX, M, TPLC, 1>(c1, v, 0);
+        fixed_stages_from<RR, EMAX, TPLC, SPLIT, 0>(c1, v);
+        st_store_global<RR::R[RR::S - 1], EMAX, M, TPLC>(c1, v);
+        if (t == 0) c.store_last(0, x0 + a0);
+    }
 }
 
 }  // namespace dfft

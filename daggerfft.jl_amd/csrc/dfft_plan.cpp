@@ -323,6 +323,7 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
     a.bsp2 = StagePlan{};
     std::vector<int> blue_fixed_R;
     int blue_fixed_tpl = 0, blue_fixed_variant = 0;
+    const Tuning *conv_tune = nullptr;
     {
         std::vector<int> f;
         std::vector<int> rader_R;
@@ -389,6 +390,7 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
                 a.bsp.L = M;
             }
             a.sp.S = 0;
+            conv_tune = tune;  // a convolution pass takes only its tile width and swizzle from the cache
             tune = nullptr;
         }
     }
@@ -441,7 +443,11 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
         std::reverse(a.bsp2.R, a.bsp2.R + a.bsp2.S);
     }
     const int emax = a.variant >= 2 ? 16 : EMAX;
-    const int lds_es = a.variant == 2 ? es / 2 : es;
+    // split exchange: half the LDS; a contiguous Rader line stages whole
+    // complex values in its slot for its permuted gather and scatter
+    // (measured, 257 FP64: 0.331 ms scattered, 0.297 ms staged; staging real
+    // and imaginary parts in two rounds through half the slot, 0.314 ms)
+    const int lds_es = (a.variant == 2 && !(a.blue == 2 && a.contig)) ? es / 2 : es;
     int tpl = a.variant ? fixed_tpl
               : (tune && tune->variant == 0 && tune->tpl > 0 && tpl_ok(L, R, emax, tune->tpl))
                   ? tune->tpl
@@ -483,7 +489,15 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
     } else {
         a.pad_shift = 30;
         // 256-byte (FP64) / 512-byte rows; the split schedule doubles the row over outer slices
-        int tc = (tune && tune->tc > 0) ? tune->tc : (a.variant >= 2 || pp.prec) ? 16 : 32;
+        int tc = (tune && tune->tc > 0) ? tune->tc : (conv_tune && conv_tune->tc > 0) ? conv_tune->tc
+                 : (a.variant >= 2 || pp.prec) ? 16 : 32;
+        // convolution tiles: the widest row segments in up to 512 threads
+        // (measured, 257-point Rader FP64: 16 -> 32 columns 2.98 -> 3.52
+        // TB/s; 800-thread tiles are slower again)
+        if (a.blue && !(conv_tune && conv_tune->tc > 0)) {
+            tc = 32;
+            while (tc > 8 && tc * tpl > 512) tc /= 2;
+        }
         int64_t cols = a.nlines;
         while (tc > 1 && tc / 2 >= cols) tc /= 2;
         while (tc > 1 && (int64_t)tc * tpl > max_threads) tc /= 2;
@@ -503,6 +517,7 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
     }
     if (pp.nblocks > 0x7fffffffLL) return DFFT_ERR_UNSUPPORTED_SIZE;
     a.swz = 1;
+    if (!tune) tune = conv_tune;
     if (tune && tune->swz > 1 && std::gcd<int64_t, int64_t>(tune->swz, pp.nblocks) == 1) a.swz = tune->swz;
     return DFFT_OK;
 }
