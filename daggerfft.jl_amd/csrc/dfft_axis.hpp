@@ -345,6 +345,61 @@ __device__ __forceinline__ void st_store_global(const C &c, const cx<typename C:
     }
 }
 
+// DCT-II (REDFT10) and DCT-III (REDFT01) along contiguous lines move the
+// reals in pairs.  FFT position p holds x[2p] for p < L/2 and x[2(L-1-p)+1]
+// for p >= L/2, so the pair (x[2m], x[2m+1]), m < L/2, feeds positions m and
+// L-1-m: one vector access per pair, unit-stride across lanes (single reals
+// at stride 2 fetch each cache line twice).  With R even and NB = L/R =
+// NBM * TPL, position m = j + r*NB (r < R/2) belongs to butterfly j = t + b*TPL
+// and position L-1-m to butterfly NB-1-j = (TPL-1-t) + (NBM-1-b)*TPL, slot
+// R-1-r: the line's mirror lane TPL-1-t, reached by a lane shuffle (a line's
+// TPL lanes are adjacent and aligned within a wavefront when TPL divides 64).
+template <int R, int EMAX, int LC, int TPLC>
+constexpr bool dct_pairs_ok() {
+    return R % 2 == 0 && LC % 2 == 0 && (LC / R) == (EMAX / R) * TPLC && TPLC <= 64 && (64 % TPLC) == 0;
+}
+
+template <int R, int EMAX, int LC, int TPLC, class T>
+__device__ __forceinline__ void st_load_dct2_pairs(const TileCtx<T, OP_DCT2, true> &c, cx<T> *v) {
+    constexpr int NBM = EMAX / R, NB = LC / R, H = R / 2;
+    const cx<T> *pairs = reinterpret_cast<const cx<T> *>(c.in) + (c.ib >> 1);
+    const bool ld = c.valid && c.t < TPLC;
+    T odd[NBM][H];
+#pragma unroll
+    for (int b = 0; b < NBM; ++b) {
+        const int j = c.t + b * TPLC;
+#pragma unroll
+        for (int r = 0; r < H; ++r) {
+            const cx<T> z = ld ? gload(pairs + j + r * NB, c.a.nt) : cx<T>{0, 0};
+            v[b * R + r] = cx<T>{z.x, (T)0};
+            odd[b][r] = z.y;
+        }
+    }
+    const int mirror = TPLC - 1 - (c.t & (TPLC - 1));
+#pragma unroll
+    for (int b = 0; b < NBM; ++b)
+#pragma unroll
+        for (int r = 0; r < H; ++r) v[b * R + R - 1 - r] = cx<T>{__shfl(odd[NBM - 1 - b][r], mirror, TPLC), (T)0};
+}
+
+template <int R, int EMAX, int LC, int TPLC, class T>
+__device__ __forceinline__ void st_store_dct3_pairs(const TileCtx<T, OP_DCT3, true> &c, const cx<T> *v) {
+    constexpr int NBM = EMAX / R, NB = LC / R, H = R / 2;
+    cx<T> *pairs = reinterpret_cast<cx<T> *>(c.out) + (c.ob >> 1);
+    const T s = (T)c.a.scale;
+    const int mirror = TPLC - 1 - (c.t & (TPLC - 1));
+#pragma unroll
+    for (int b = 0; b < NBM; ++b) {
+        const int j = c.t + b * TPLC;
+#pragma unroll
+        for (int r = 0; r < H; ++r) {
+            // position L-1-m from the mirror lane (every lane takes part in the shuffle)
+            const T w = __shfl(v[(NBM - 1 - b) * R + R - 1 - r].x, mirror, TPLC);
+            if (c.valid && c.t < TPLC) gstore(pairs + j + r * NB, cx<T>{v[b * R + r].x * s, w * s}, c.a.nt);
+        }
+    }
+}
+
 // Stockham write of stage outputs: y[(j - j mod Ns) R + j mod Ns + r Ns]
 template <int R, int EMAX, int PART, int LC, int TPLC, int Ns, class C>
 __device__ __forceinline__ void st_write_lds(const C &c, const cx<typename C::value_type> *v) {
@@ -586,13 +641,17 @@ __device__ __forceinline__ void r2c_post_split(const TileCtx<T, OP, CONTIG> &c, 
 // All stages of a fixed schedule: global -> registers, LDS exchanges, registers -> global.
 template <class RL, int EMAX, int TPLC, bool SPLIT, class T, int OP, bool CONTIG>
 __device__ __forceinline__ void run_fixed_stages(const TileCtx<T, OP, CONTIG> &c) {
-    constexpr int L = RL::L;
+    constexpr int L = RL::L, R0 = RL::R[0], RN = RL::R[RL::S - 1];
     cx<T> v[EMAX];
-    st_load_global<RL::R[0], EMAX, L, TPLC>(c, v);
-    st_compute<RL::R[0], EMAX, L, TPLC, 1>(c, v, 0);
+    if constexpr (OP == OP_DCT2 && CONTIG && dct_pairs_ok<R0, EMAX, L, TPLC>())
+        st_load_dct2_pairs<R0, EMAX, L, TPLC>(c, v);
+    else st_load_global<R0, EMAX, L, TPLC>(c, v);
+    st_compute<R0, EMAX, L, TPLC, 1>(c, v, 0);
     fixed_stages_from<RL, EMAX, TPLC, SPLIT, 0>(c, v);
-    if constexpr (OP == OP_R2C && SPLIT) r2c_post_split<RL::R[RL::S - 1], EMAX, L, TPLC>(c, v);
-    else st_store_global<RL::R[RL::S - 1], EMAX, L, TPLC>(c, v);
+    if constexpr (OP == OP_R2C && SPLIT) r2c_post_split<RN, EMAX, L, TPLC>(c, v);
+    else if constexpr (OP == OP_DCT3 && CONTIG && dct_pairs_ok<RN, EMAX, L, TPLC>())
+        st_store_dct3_pairs<RN, EMAX, L, TPLC>(c, v);
+    else st_store_global<RN, EMAX, L, TPLC>(c, v);
 }
 
 // Runs tile `tile` of pass `a` with the calling workgroup.  Threads with
