@@ -18,10 +18,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
 #include <numeric>
+#include <string>
 #include <vector>
 
 #include "../../include/daggerfft_hip.h"
@@ -482,7 +485,8 @@ static int lanes_create(dfft_batch *b, uint32_t flags) {
     for (int t = 0; t < b->count; ++t) {
         const dfft_batch_desc &d = b->descs[t];
         dfft_plan *p = nullptr;
-        const uint32_t pf = (d.flags & DFFT_FLAG_UNNORMALIZED) | (flags & DFFT_FLAG_NO_TUNE);
+        // (DFFT_FLAG_MEASURE: each transform's untuned passes are timed first)
+        const uint32_t pf = (d.flags & DFFT_FLAG_UNNORMALIZED) | (flags & (DFFT_FLAG_NO_TUNE | DFFT_FLAG_MEASURE));
         int rc = dfft_plan_create(&p, 3, d.n, d.kind, d.sign, nullptr, d.precision, DFFT_PENCIL, dims, 3, pf);
         if (rc != DFFT_OK) return rc;
         b->plans.push_back(p);
@@ -570,6 +574,95 @@ static int lanes_execute(dfft_batch *b, void *const *d_in, void *const *d_out, h
     return DFFT_OK;
 }
 
+// The schedule cache key of a batch's launch schedule: its transforms'
+// descriptors (sizes, kind, sign, precision, flags), FNV-1a hashed.
+static std::string batch_key(int count, const dfft_batch_desc *descs) {
+    uint64_t h = 1469598103934665603ULL;
+    auto mix = [&](int64_t v) {
+        for (int k = 0; k < 8; ++k) {
+            h ^= (uint64_t)((v >> (8 * k)) & 0xff);
+            h *= 1099511628211ULL;
+        }
+    };
+    for (int t = 0; t < count; ++t) {
+        const dfft_batch_desc &d = descs[t];
+        for (int a = 0; a < 3; ++a) mix(d.n[a]);
+        mix(d.kind);
+        mix(d.sign);
+        mix(d.precision);
+        mix(d.flags);
+    }
+    char buf[64];
+    snprintf(buf, sizeof buf, "batch/%d/%016llx", count, (unsigned long long)h);
+    return buf;
+}
+
+// Launch-schedule values of a batch's cache entry (Tuning::variant)
+constexpr int BATCH_PICK_GROUPED = 1, BATCH_PICK_LANES = 2;
+
+extern "C" int dfft_batch_create(dfft_batch **batch, int count, const dfft_batch_desc *descs, uint32_t flags);
+
+// DFFT_FLAG_MEASURE: times the grouped schedule (when every pass fits it) and
+// the lanes schedule (each transform's own plan, its passes autotuned) on
+// scratch arrays of the batch's sizes, best of three, and returns the faster
+// (BATCH_PICK_*), or a negative DFFT_ERR_* code.
+static int measure_batch(int count, const dfft_batch_desc *descs, uint32_t flags) {
+    const uint32_t base = flags & ~(uint32_t)(DFFT_FLAG_GRAPH | DFFT_FLAG_BATCH_GROUPED | DFFT_FLAG_BATCH_LANES);
+    std::vector<void *> in(count, nullptr), out(count, nullptr);
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = DFFT_OK;
+    for (int t = 0; t < count && rc == DFFT_OK; ++t) {
+        const dfft_batch_desc &d = descs[t];
+        const int64_t N = d.n[0] * d.n[1] * d.n[2];
+        const int64_t NC = (d.kind == DFFT_R2C ? d.n[0] / 2 + 1 : d.n[0]) * d.n[1] * d.n[2];
+        const size_t es = d.precision ? 8 : 4;
+        const size_t ib = (size_t)N * es * (d.kind == DFFT_R2C ? 1 : 2), ob = (size_t)NC * es * 2;
+        if (hipMalloc(&in[t], ib) != hipSuccess || hipMalloc(&out[t], ob) != hipSuccess) rc = DFFT_ERR_OOM;
+        else if (hipError_t e = launch_fill(in[t], (int64_t)(ib / es), d.precision, 17 + t, nullptr))
+            rc = hip_fail(e, "input fill");
+    }
+    if (rc == DFFT_OK) {
+        if (hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) rc = hip_fail(e, "hipStreamCreate");
+        else if (hipError_t e2 = hipEventCreate(&e0)) rc = hip_fail(e2, "hipEventCreate");
+        else if (hipError_t e3 = hipEventCreate(&e1)) rc = hip_fail(e3, "hipEventCreate");
+    }
+    if (rc == DFFT_OK && hipDeviceSynchronize() != hipSuccess) rc = hip_fail(hipGetLastError(), "fill");
+    float best[2] = {-1.0f, -1.0f};
+    for (int k = 0; k < 2 && rc == DFFT_OK; ++k) {
+        dfft_batch *b = nullptr;
+        const uint32_t sf = base | (k == 0 ? DFFT_FLAG_BATCH_GROUPED : DFFT_FLAG_BATCH_LANES);
+        const int crc = dfft_batch_create(&b, count, descs, sf);
+        if (crc != DFFT_OK) {
+            if (k == 1) rc = crc;  // lanes always builds; a grouped batch may not
+            continue;
+        }
+        bool ok = dfft_batch_execute(b, in.data(), out.data(), st) == DFFT_OK;  // warm-up
+        float m = 1e30f;
+        for (int trial = 0; trial < 3 && ok; ++trial) {
+            ok = hipEventRecord(e0, st) == hipSuccess;
+            for (int r = 0; r < 2 && ok; ++r) ok = dfft_batch_execute(b, in.data(), out.data(), st) == DFFT_OK;
+            ok = ok && hipEventRecord(e1, st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+            float ms = 0;
+            if (ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) m = std::min(m, ms / 2);
+        }
+        if (ok) best[k] = m;
+        else rc = hip_fail(hipGetLastError(), "batch timing");
+        dfft_batch_destroy(b);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    for (int t = 0; t < count; ++t) {
+        if (in[t]) (void)hipFree(in[t]);
+        if (out[t]) (void)hipFree(out[t]);
+    }
+    if (rc != DFFT_OK) return -rc;
+    if (getenv("DFFT_TUNE_VERBOSE"))
+        fprintf(stderr, "tune batch of %d: grouped %.4f ms, lanes %.4f ms\n", count, best[0], best[1]);
+    return (best[0] >= 0 && best[0] <= best[1]) ? BATCH_PICK_GROUPED : BATCH_PICK_LANES;
+}
+
 extern "C" {
 
 int dfft_batch_create(dfft_batch **batch, int count, const dfft_batch_desc *descs, uint32_t flags) {
@@ -596,11 +689,32 @@ int dfft_batch_create(dfft_batch **batch, int count, const dfft_batch_desc *desc
     }
     bool grouped = (flags & DFFT_FLAG_BATCH_GROUPED) != 0;
     if (!grouped && !(flags & DFFT_FLAG_BATCH_LANES)) {
-        // default: grouped when every pass has a compile-time grouped schedule
-        grouped = true;
-        for (int t = 0; t < count && grouped; ++t) {
-            grouped = descs[t].kind == DFFT_C2C;
-            for (int i = 0; i < 3 && grouped; ++i) grouped = gfix_lookup(descs[t].precision, 1, (int)descs[t].n[i]) >= 0;
+        // default: the schedule cache's pick for this mix (DFFT_FLAG_MEASURE
+        // times both schedules when there is none); without one, grouped
+        // when every pass has a compile-time grouped schedule
+        const std::string key = batch_key(count, descs);
+        Tuning pick;
+        bool have = !(flags & DFFT_FLAG_NO_TUNE) && tuning_lookup_key(key, pick);
+        if (!have && (flags & DFFT_FLAG_MEASURE) && !(flags & DFFT_FLAG_NO_TUNE)) {
+            const int m = measure_batch(count, descs, flags);
+            if (m < 0) {
+                batch_free(b);
+                return -m;
+            }
+            pick = Tuning{};
+            pick.variant = m;
+            tuning_store_key(key, pick);
+            have = true;
+        }
+        if (have) {
+            grouped = pick.variant == BATCH_PICK_GROUPED;
+        } else {
+            grouped = true;
+            for (int t = 0; t < count && grouped; ++t) {
+                grouped = descs[t].kind == DFFT_C2C;
+                for (int i = 0; i < 3 && grouped; ++i)
+                    grouped = gfix_lookup(descs[t].precision, 1, (int)descs[t].n[i]) >= 0;
+            }
         }
     }
     if (!grouped) {

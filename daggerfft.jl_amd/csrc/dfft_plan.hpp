@@ -69,6 +69,10 @@ struct TwiddleTable {
     int upload(void **d_ptr, size_t *bytes) const;
 };
 bool tuning_lookup(const PassPlan &pp, Tuning &t);
+// schedule cache entries under a key of their own (a batch's launch schedule);
+// storing saves the cache when a save path is set
+bool tuning_lookup_key(const std::string &key, Tuning &t);
+void tuning_store_key(const std::string &key, const Tuning &t);
 // Writes `entries` as a stamped wisdom file.
 int wisdom_write(const char *path, const std::vector<std::pair<std::string, Tuning>> &entries);
 // DFFT_FLAG_MEASURE: tune the passes of this plan that have no cache entry.

@@ -159,6 +159,19 @@ static void tuning_store(const std::string &key, const Tuning &t) {
     g_cache[key] = t;
 }
 
+bool tuning_lookup_key(const std::string &key, Tuning &t) {
+    std::lock_guard<std::mutex> g(g_mu);
+    auto it = g_cache.find(key);
+    if (it == g_cache.end()) return false;
+    t = it->second;
+    return true;
+}
+
+void tuning_store_key(const std::string &key, const Tuning &t) {
+    tuning_store(key, t);
+    autosave();
+}
+
 // Radix orders to try for length L: all splits of the power-of-two part
 // into 8/4/2 factors (as multisets), each in descending and ascending order,
 // followed/preceded by the odd factors.

@@ -8,7 +8,9 @@ saves the fastest schedule of every axis pass.
 Shapes: 512^3 ComplexF64 (full search), 128^3 ComplexF64, 256^3 R2C and C2R Float64,
 384x240x210 ComplexF64, the DCT-II/III configurations, and every distinct
 request shape of the mixed serving trace (cubic 64..512, C2C and R2C, both
-precisions).  Progress is printed per shape.
+precisions); then the launch schedule of the 64-transform mixed batch
+(grouped against lanes, with the lanes' passes tuned).  Progress is printed
+per shape.
 """
 import os
 import sys
@@ -47,6 +49,12 @@ def main():
         print("[%d/%d] %s %s %s  %.1fs  entries %d" % (i + 1, len(jobs), "x".join(map(str, shape)),
                                                      ("C2C", "R2C", "C2R", "R2R")[kind], "f64" if prec else "f32",
                                                      time.time() - t0, D.tune_entries()), flush=True)
+    from daggerfft.batch import Batch, batch_config_sizes
+    descs = [(shape, L.C2C, -1, prec) for shape, prec in batch_config_sizes(64, 7)]
+    b = Batch(descs, flags=L.FLAG_MEASURE)
+    print("batch of 64: %s schedule  %.1fs  entries %d" % ("grouped" if b.launches == 3 else "lanes",
+                                                          time.time() - t0, D.tune_entries()), flush=True)
+    b.destroy()
     if out is None:
         import subprocess
         arch = subprocess.run(["rocminfo"], capture_output=True, text=True).stdout

@@ -99,8 +99,9 @@ extern "C" {
 #define DFFT_FLAG_BATCH_GROUPED  0x80u /* batch: one grouped launch per axis-pass level (three launches) */
 #define DFFT_FLAG_BATCH_LANES    0x100u /* batch: each transform's own plan on four forked streams */
 #define DFFT_FLAG_MEASURE        0x200u /* time the schedules of passes the cache lacks at plan
-                                         * creation (FFTW.MEASURE, src/fft.jl:127); synchronises
-                                         * the device and saves the cache if dfft_tune_autosave is set */
+                                         * creation (FFTW.MEASURE, src/fft.jl:127), and a batch's
+                                         * launch schedule; synchronises the device and saves the
+                                         * cache if dfft_tune_autosave is set */
 
 typedef struct dfft_plan dfft_plan;
 
@@ -279,10 +280,13 @@ int dfft_plan_export_wisdom(const dfft_plan *plan, const char *path);
  *                            kernels) on one of four streams forked from and
  *                            joined back into the caller's stream; results
  *                            equal each transform run alone bit for bit.
- * Neither flag: grouped when every axis length of every transform is one of
- * the compile-time grouped lengths and every transform is C2C (the
- * 64-transform mix of 32..256: 2.15 ms grouped against 2.20 ms lanes), else
- * lanes.  Both flags: DFFT_ERR_ARG.
+ * Neither flag: the schedule cache's pick for this mix of descriptors; with
+ * DFFT_FLAG_MEASURE and no pick cached, both schedules are timed on scratch
+ * arrays of the batch's sizes (the lanes' plans with their passes
+ * autotuned) and the faster is cached (and saved with the cache when a save
+ * path is set).  With no pick: grouped when every axis length of every
+ * transform is one of the compile-time grouped lengths and every transform
+ * is C2C, else lanes.  Both flags: DFFT_ERR_ARG.
  * With DFFT_FLAG_GRAPH at creation the launches are recorded once as a
  * hipGraph (the lanes as parallel branches) and replayed by every execute,
  * re-recorded when the buffer pointers change. */

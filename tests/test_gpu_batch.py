@@ -296,3 +296,42 @@ def test_batch_default_schedule(gpu):
     h = ctypes.c_void_p()
     st = L.lib().dfft_batch_create(ctypes.byref(h), 1, arr, L.FLAG_BATCH_GROUPED | L.FLAG_BATCH_LANES)
     assert st == L.DFFT_ERR_ARG
+
+
+def test_measured_batch_schedule_is_cached(gpu, tmp_path):
+    """DFFT_FLAG_MEASURE on a batch with no schedule flag times the grouped
+    and lanes schedules and caches the faster under the mix's key: a second
+    batch of the same mix without the flag takes the cached schedule, the
+    saved cache holds the entry, and the results equal the oracle's."""
+    import daggerfft as D
+    from daggerfft import _lib as L
+    from daggerfft.batch import Batch
+    descs = [((64, 48, 32), L.C2C, -1, 1), ((32, 32, 96), L.C2C, -1, 0), ((96, 64, 48), L.C2C, +1, 1),
+             ((48, 48, 48), L.C2C, -1, 0), ((128, 32, 64), L.C2C, -1, 1)]
+    import os
+    shipped = os.path.join(os.path.dirname(D.__file__), "..", "wisdom", "gfx950.txt")
+    D.tune_clear()
+    D.tune_load(shipped)
+    try:
+        n0 = D.tune_entries()
+        b = Batch(descs, flags=L.FLAG_MEASURE)
+        picked = b.launches
+        assert picked in (3, 15), picked  # grouped: 3 launches; lanes: 3 per transform
+        assert D.tune_entries() > n0
+        again = Batch(descs)
+        assert again.launches == picked
+        path = str(tmp_path / "wisdom.txt")
+        D.tune_save(path)
+        assert any(line.startswith("batch/5/") for line in open(path))
+        ins = [D.rand(shape, np.complex128 if prec == 1 else np.complex64, seed=70 + t)
+               for t, (shape, _, _, prec) in enumerate(descs)]
+        outs = [X.similar() for X in ins]
+        again.execute(ins, outs)
+        for t, ((shape, kind, sign, prec), Y) in enumerate(zip(descs, outs)):
+            dt = np.complex128 if prec == 1 else np.complex64
+            a = O.fill(70 + t, shape, dt).astype(np.complex128)
+            e = O.rel_l2(Y.to_host().astype(np.complex128), O.fftn(a, inverse=sign > 0))
+            assert e <= (1e-12 if prec == 1 else 1e-5), (t, e)
+    finally:
+        D.tune_clear()
+        D.tune_load(shipped)

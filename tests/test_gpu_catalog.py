@@ -187,25 +187,41 @@ def test_catalog_pow2_alternative_radix_lists(gpu, tmp_path, prec):
                 D.tune_clear()
 
 
+@pytest.mark.parametrize("variant", [2])
 @pytest.mark.parametrize("prec", [1, 0])
-def test_catalog_dct_pow2(gpu, prec):
-    """DCT-II / DCT-III catalog kernels (split LDS) of the powers of two
-    16..1024, along the contiguous and a strided axis, against the oracle."""
+def test_catalog_dct_pow2(gpu, prec, variant, tmp_path):
+    """DCT-II / DCT-III catalog kernels (split 16-value schedule, forced
+    through the schedule cache) of the powers of two 16..1024, along the
+    contiguous axis (paired loads and stores) and a strided axis, on 2 and 6
+    lines, against the oracle."""
     import daggerfft as D
     from daggerfft import _lib as L
-    D.tune_clear()
     dt = np.float64 if prec else np.float32
     tol = 1e-12 if prec else 1e-5
+    cases = []
     for n in (16, 32, 64, 128, 256, 512, 1024):
-        for shape, ax in (((n, 3, 2), 0), ((2, n, 3), 1)):
+        for shape, ax, layout in (((n, 3, 2), 0, 1), ((n, 2, 1), 0, 1), ((2, n, 3), 1, 2)):
+            lines = int(np.prod(shape)) // n
+            cases.append((n, shape, ax, layout, int(math.floor(math.log2(lines * n)))))
+    path = str(tmp_path / "dct.txt")
+    with open(path, "w") as f:
+        for n, shape, ax, layout, sc in cases:
+            for op in (5, 6):  # DCT2, DCT3
+                f.write("%d/%d/%d/%d/%d - 0 0 0 %d 1\n" % (prec, op, n, layout, sc, variant))
+    D.tune_clear()
+    D.tune_load(path)
+    try:
+        for n, shape, ax, layout, sc in cases:
             a = O.fill(n + ax, shape, dt)
             for kind, okind in ((D.REDFT10, O.REDFT10), (D.REDFT01, O.REDFT01)):
                 p = D.Plan(shape, L.R2R, -1, prec, D.Pencil(), (ax + 1,), r2r_kinds=(kind,))
                 q = p.passes()[0]
-                assert q["variant"] == 2 and q["op"] in ("DCT2", "DCT3"), q
+                assert q["variant"] == variant and q["op"] in ("DCT2", "DCT3"), q
                 G = p.execute(D.DeviceArray.from_host(a), D.DeviceArray(shape, dt))
                 e = O.rel_l2(G.to_host().astype(np.float64), O.r2rn(a.astype(np.float64), okind, (ax + 1,)))
-                assert e <= tol, (n, ax, kind, e)
+                assert e <= tol, (n, shape, ax, kind, e)
+    finally:
+        D.tune_clear()
 
 
 @pytest.mark.parametrize("swz", [97, 1021])
