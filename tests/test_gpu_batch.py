@@ -278,16 +278,22 @@ def test_lanes_batch_equals_plans_bitwise(gpu, flags):
 
 
 def test_batch_default_schedule(gpu):
-    """Without a schedule flag the batch runs grouped (three launches) when
-    every transform is C2C and every axis length has a compile-time grouped
-    schedule, else lanes (three launches per transform); both flags at once
-    are refused."""
+    """Without a schedule flag a batch takes the schedule cache's measured
+    pick for its mix (the shipped cache picks lanes for the 64-transform
+    mix); with no pick it runs grouped (three launches) when every transform
+    is C2C and every axis length has a compile-time grouped schedule, else
+    lanes (three launches per transform); both flags at once are refused."""
     import ctypes
+    import os
 
+    import daggerfft as D
     from daggerfft import _lib as L
     from daggerfft.batch import Batch, batch_config_sizes
+    D.tune_load(os.path.join(os.path.dirname(D.__file__), "..", "wisdom", "gfx950.txt"))
     cfg = batch_config_sizes(64, 7)
-    assert Batch([(shape, L.C2C, -1, prec) for shape, prec in cfg]).launches == 3
+    mix = [(shape, L.C2C, -1, prec) for shape, prec in cfg]
+    assert Batch(mix).launches == 192
+    assert Batch(mix, flags=L.FLAG_NO_TUNE).launches == 3
     assert Batch([((64, 30, 48), L.C2C, -1, 1), ((32, 32, 32), L.C2C, -1, 0)]).launches == 6
     assert Batch([((64, 64, 64), L.R2C, -1, 1), ((32, 32, 32), L.C2C, -1, 0)]).launches == 6
     arr = (L.dfft_batch_desc * 1)()
