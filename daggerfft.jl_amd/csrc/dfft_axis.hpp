@@ -164,6 +164,29 @@ struct TileCtx {
             if (p == 0 || p == n + 1) return cx<T>{0, 0};
             if (p <= n) return cx<T>{gload(inr + ib + (int64_t)(p - 1) * a.in_sm, a.nt), (T)0};
             return cx<T>{-gload(inr + ib + (int64_t)(a.sp.L - p - 1) * a.in_sm, a.nt), (T)0};
+        } else if constexpr (OP == OP_DCT2H) {
+            // z_p = v_2p + i v_2p+1 of Makhoul's v_i = x[2i] (i < H), x[2n-2i-1] (i >= H)
+            if (!valid) return cx<T>{0, 0};
+            const int H = a.sp.L, n = 2 * H;
+            auto v = [&](int i) {
+                const int q = i < H ? 2 * i : 2 * n - 2 * i - 1;
+                return gload(inr + ib + (int64_t)q * a.in_sm, a.nt);
+            };
+            return cx<T>{v(2 * p), v(2 * p + 1)};
+        } else if constexpr (OP == OP_DCT3H) {
+            // U_k = e^{+i pi k/2n} (X_k - i X_{n-k}) (X_n = 0) is the spectrum of
+            // Makhoul's v; Z_p = (U_p + conj U_{H-p}) + i e^{2 pi i p/n} (U_p -
+            // conj U_{H-p}) that of z_j = v_2j + i v_2j+1 (the inverse H-point FFT
+            // runs as a forward FFT on conjugates).  Each X_k is read by two
+            // positions: cached loads.
+            if (!valid) return cx<T>{0, 0};
+            const int H = a.sp.L, n = 2 * H;
+            const cx<T> *pw = post(), *ww = post() + H + 1;
+            auto X = [&](int k) { return k >= n ? (T)0 : gload(inr + ib + (int64_t)k * a.in_sm, false); };
+            auto U = [&](int k) { return cmul(conj(ww[k]), cx<T>{X(k), -X(n - k)}); };
+            const cx<T> Up = U(p), Uh = U(H - p);
+            const cx<T> A = Up + conj(Uh), B = Up - conj(Uh);
+            return conj(A + mul_pos_i(cmul(conj(pw[p]), B)));
         } else {  // OP_DCT4, OP_DST4: x_p e^{-i pi p / 2n}, zero-padded to 2n
             if (!valid || p >= a.nlog) return cx<T>{0, 0};
             T x = gload(inr + ib + (int64_t)p * a.in_sm, a.nt);
@@ -176,8 +199,18 @@ struct TileCtx {
         T *outr = reinterpret_cast<T *>(out);
         cx<T> *outc = reinterpret_cast<cx<T> *>(out);
         const T s = (T)a.scale;
-        if constexpr (OP == OP_R2C) {
+        if constexpr (OP == OP_R2C || OP == OP_DCT2H) {
             lds[li(p)] = v;  // post-processed after a barrier
+        } else if constexpr (OP == OP_DCT3H) {
+            // z_p = conj(v) = v_2p + i v_2p+1, v_i at x[2i] (i < H), x[2n-2i-1] (i >= H)
+            if (!valid) return;
+            const int H = a.sp.L, n = 2 * H;
+            auto put = [&](int i, T y) {
+                const int q = i < H ? 2 * i : 2 * n - 2 * i - 1;
+                gstore(outr + ob + (int64_t)q * a.out_sm, y * s, a.nt);
+            };
+            put(2 * p, v.x);
+            put(2 * p + 1, -v.y);
         } else {
             if (!valid) return;
             if constexpr (OP == OP_C2C) {
@@ -396,6 +429,85 @@ __device__ __forceinline__ void st_store_dct3_pairs(const TileCtx<T, OP_DCT3, tr
             // position L-1-m from the mirror lane (every lane takes part in the shuffle)
             const T w = __shfl(v[(NBM - 1 - b) * R + R - 1 - r].x, mirror, TPLC);
             if (c.valid && c.t < TPLC) gstore(pairs + j + r * NB, cx<T>{v[b * R + r].x * s, w * s}, c.a.nt);
+        }
+    }
+}
+
+// The half-length DCTs along contiguous lines move the reals in quads: with
+// n = 2H and H even, x[4m..4m+3] hold v_2m, v_n-1-2m, v_2m+1, v_n-2-2m, that
+// is z_m = (x[4m], x[4m+2]) and z_{H-1-m} = (x[4m+3], x[4m+1]) (m < H/2):
+// positions m and H-1-m of the H-point FFT, on mirror lanes as above.
+template <int R, int EMAX, int LC, int TPLC, class T>
+__device__ __forceinline__ void st_load_dct2h_quads(const TileCtx<T, OP_DCT2H, true> &c, cx<T> *v) {
+    constexpr int NBM = EMAX / R, NB = LC / R, HR = R / 2;
+    const cx<T> *pairs = reinterpret_cast<const cx<T> *>(c.in) + (c.ib >> 1);
+    const bool ld = c.valid && c.t < TPLC;
+    cx<T> far[NBM][HR];
+#pragma unroll
+    for (int b = 0; b < NBM; ++b) {
+        const int j = c.t + b * TPLC;
+#pragma unroll
+        for (int r = 0; r < HR; ++r) {
+            const int m = j + r * NB;
+            const cx<T> lo = ld ? gload(pairs + 2 * m, c.a.nt) : cx<T>{0, 0};
+            const cx<T> hi = ld ? gload(pairs + 2 * m + 1, c.a.nt) : cx<T>{0, 0};
+            v[b * R + r] = cx<T>{lo.x, hi.x};
+            far[b][r] = cx<T>{hi.y, lo.y};
+        }
+    }
+    const int mirror = TPLC - 1 - (c.t & (TPLC - 1));
+#pragma unroll
+    for (int b = 0; b < NBM; ++b)
+#pragma unroll
+        for (int r = 0; r < HR; ++r)
+            v[b * R + R - 1 - r] = cx<T>{__shfl(far[NBM - 1 - b][r].x, mirror, TPLC),
+                                         __shfl(far[NBM - 1 - b][r].y, mirror, TPLC)};
+}
+
+template <int R, int EMAX, int LC, int TPLC, class T>
+__device__ __forceinline__ void st_store_dct3h_quads(const TileCtx<T, OP_DCT3H, true> &c, const cx<T> *v) {
+    constexpr int NBM = EMAX / R, NB = LC / R, HR = R / 2;
+    cx<T> *pairs = reinterpret_cast<cx<T> *>(c.out) + (c.ob >> 1);
+    const T s = (T)c.a.scale;
+    const int mirror = TPLC - 1 - (c.t & (TPLC - 1));
+#pragma unroll
+    for (int b = 0; b < NBM; ++b) {
+        const int j = c.t + b * TPLC;
+#pragma unroll
+        for (int r = 0; r < HR; ++r) {
+            // position m = j + r*NB here, H-1-m on the mirror lane; z = conj(v)
+            const cx<T> w = v[(NBM - 1 - b) * R + R - 1 - r];
+            const cx<T> f{__shfl(w.x, mirror, TPLC), __shfl(w.y, mirror, TPLC)};
+            const cx<T> o = v[b * R + r];
+            if (c.valid && c.t < TPLC) {
+                const int m = j + r * NB;
+                gstore(pairs + 2 * m, cx<T>{o.x * s, -f.y * s}, c.a.nt);
+                gstore(pairs + 2 * m + 1, cx<T>{-o.y * s, f.x * s}, c.a.nt);
+            }
+        }
+    }
+}
+
+// DCT-II outputs from the H-point FFT Z of z_m = v_2m + i v_2m+1 in LDS:
+// V_k = 1/2 [(Z_k + conj Z_{H-k}) - i e^{-2 pi i k/n} (Z_k - conj Z_{H-k})]
+// (k = 0..H) is the n-point DFT of Makhoul's v, and with u = e^{-i pi k/2n} V_k
+// the DCT-II is X_k = 2 Re u and X_{n-k} = -2 Im u.
+template <class T, bool CONTIG>
+__device__ __forceinline__ void dct2h_post_lds(const TileCtx<T, OP_DCT2H, CONTIG> &c, int TPL) {
+    __syncthreads();
+    const int H = c.a.sp.L, n = 2 * H;
+    const T s = (T)c.a.scale;
+    T *outr = reinterpret_cast<T *>(c.out);
+    const cx<T> *pw = c.post(), *ww = c.post() + H + 1;
+    for (int k = c.t; k <= H; k += TPL) {
+        const cx<T> zk = c.lds[c.li(k == H ? 0 : k)];
+        const cx<T> zc = conj(c.lds[c.li(k == 0 ? 0 : H - k)]);
+        const cx<T> A = zk + zc, B = zk - zc;
+        const cx<T> V = scl(A + mul_neg_i(cmul(pw[k], B)), (T)0.5);
+        const cx<T> u = cmul(ww[k], V);
+        if (c.valid) {
+            gstore(outr + c.ob + (int64_t)k * c.a.out_sm, (T)2 * u.x * s, c.a.nt);
+            if (k > 0 && k < H) gstore(outr + c.ob + (int64_t)(n - k) * c.a.out_sm, (T)-2 * u.y * s, c.a.nt);
         }
     }
 }
@@ -645,12 +757,16 @@ __device__ __forceinline__ void run_fixed_stages(const TileCtx<T, OP, CONTIG> &c
     cx<T> v[EMAX];
     if constexpr (OP == OP_DCT2 && CONTIG && dct_pairs_ok<R0, EMAX, L, TPLC>())
         st_load_dct2_pairs<R0, EMAX, L, TPLC>(c, v);
+    else if constexpr (OP == OP_DCT2H && CONTIG && dct_pairs_ok<R0, EMAX, L, TPLC>())
+        st_load_dct2h_quads<R0, EMAX, L, TPLC>(c, v);
     else st_load_global<R0, EMAX, L, TPLC>(c, v);
     st_compute<R0, EMAX, L, TPLC, 1>(c, v, 0);
     fixed_stages_from<RL, EMAX, TPLC, SPLIT, 0>(c, v);
     if constexpr (OP == OP_R2C && SPLIT) r2c_post_split<RN, EMAX, L, TPLC>(c, v);
     else if constexpr (OP == OP_DCT3 && CONTIG && dct_pairs_ok<RN, EMAX, L, TPLC>())
         st_store_dct3_pairs<RN, EMAX, L, TPLC>(c, v);
+    else if constexpr (OP == OP_DCT3H && CONTIG && dct_pairs_ok<RN, EMAX, L, TPLC>())
+        st_store_dct3h_quads<RN, EMAX, L, TPLC>(c, v);
     else st_store_global<RN, EMAX, L, TPLC>(c, v);
 }
 
@@ -757,6 +873,7 @@ __device__ __forceinline__ void axis_tile(const AxisPass &a, const void *in, voi
     }
 
     if constexpr (OP == OP_R2C && !(FIXED && SPLIT)) r2c_post_lds(c, TPL);
+    if constexpr (OP == OP_DCT2H) dct2h_post_lds(c, TPL);
 }
 
 // ---------------------------------------------------------------------------

@@ -49,12 +49,12 @@ static axis_kernel_t g_axis_kernels[2][2][OP_COUNT] = {
 // ---------------------------------------------------------------------------
 extern const FixedTable g_fixed_pow2, g_fixed_f64a, g_fixed_f64b, g_fixed_f64c, g_fixed_f32a, g_fixed_f32b,
     g_fixed_f32c, g_fixed_r2ca, g_fixed_r2cb, g_fixed_c2ra, g_fixed_c2rb, g_fixed_dct, g_fixed_long, g_fixed_mid64a,
-    g_fixed_mid64b, g_fixed_mid32a, g_fixed_mid32b, g_fixed_reala, g_fixed_realb, g_fixed_r2c32;
+    g_fixed_mid64b, g_fixed_mid32a, g_fixed_mid32b, g_fixed_reala, g_fixed_realb, g_fixed_r2c32, g_fixed_dcth;
 static const FixedTable *const g_fixed_tables[] = {
     &g_fixed_pow2,  &g_fixed_f64a,  &g_fixed_f64b,   &g_fixed_f64c,   &g_fixed_f32a,   &g_fixed_f32b,
     &g_fixed_f32c,  &g_fixed_r2ca,  &g_fixed_r2cb,   &g_fixed_c2ra,   &g_fixed_c2rb,   &g_fixed_dct,
     &g_fixed_long,  &g_fixed_mid64a, &g_fixed_mid64b, &g_fixed_mid32a, &g_fixed_mid32b, &g_fixed_reala,
-    &g_fixed_realb, &g_fixed_r2c32};
+    &g_fixed_realb, &g_fixed_r2c32, &g_fixed_dcth};
 
 // The catalog kernel of (precision, layout, variant, op, length) with radix
 // list R[0..S) -- or, with R == nullptr, the first one listed (the default).
@@ -101,6 +101,7 @@ static axis_kernel_t axis_kernel(const AxisPass &a, int prec) {
         const FixedEntry *e = fixed_entry(prec, a.contig, a.variant, a.op, a.sp.L, a.sp.R, a.sp.S);
         return e ? e->fn : nullptr;
     }
+    if (a.op < 0 || a.op >= OP_COUNT) return nullptr;  // catalog-only ops
     return g_axis_kernels[prec ? 1 : 0][a.contig ? 1 : 0][a.op];
 }
 

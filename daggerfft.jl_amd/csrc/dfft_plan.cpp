@@ -105,6 +105,11 @@ static void build_twiddles(StagePlan &sp, int op, int nlog, std::vector<long dou
         // exp(-i pi k / 2L), k = 0..L-1
         sp.post = (int)(tab.size() / 2);
         for (int k = 0; k < sp.L; ++k) push_w(tab, k, 4 * (int64_t)sp.L);
+    } else if (op == OP_DCT2H || op == OP_DCT3H) {
+        // e^{-2 pi i k/n} then e^{-i pi k/2n}, k = 0..H (n = 2H = 2 sp.L)
+        sp.post = (int)(tab.size() / 2);
+        for (int k = 0; k <= sp.L; ++k) push_w(tab, k, 2 * (int64_t)sp.L);
+        for (int k = 0; k <= sp.L; ++k) push_w(tab, k, 8 * (int64_t)sp.L);
     } else if (op == OP_DCT4 || op == OP_DST4) {
         // exp(-2 pi i m / 4L), m = 0..L-1 (L = 2n): pre-twiddle e^{-i pi p/2n} at
         // m = 2p, post-twiddle e^{-i pi (2k+1)/4n} at m = 2k+1
@@ -428,6 +433,7 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
         fixed_tpl = blue_fixed_tpl;
     }
     if (!a.variant) {
+        if (a.op >= OP_COUNT) return DFFT_ERR_UNSUPPORTED_SIZE;  // catalog-only ops
         // a tuned radix list for the runtime engine (radices up to 8); a
         // catalog kernel's list (variant >= 1) may hold composite radices
         if (tune && tune->variant == 0 && !tune->radices.empty()) R = tune->radices;
@@ -608,6 +614,8 @@ static const char *op_name(int op) {
     case OP_DST1: return "DST1";
     case OP_DCT4: return "DCT4";
     case OP_DST4: return "DST4";
+    case OP_DCT2H: return "DCT2H";
+    case OP_DCT3H: return "DCT3H";
     }
     return "?";
 }
@@ -733,6 +741,26 @@ int build_plan(dfft_plan *p) {
             }
             const double sc = (p->flags & DFFT_FLAG_NORMALIZE_R2R) ? 1.0 / r2r_logical(kinds[i], nl) : 1.0;
             add(ax, op, L, nl, n, n, i == 0 ? BUF_IN : BUF_OUT, BUF_OUT, sc, false, 2.0 * N * es_r);
+            // REDFT10 / REDFT01 of n = 4k reals through an n/2-point complex FFT
+            // where the catalog has it: half the FFT work per line, half the
+            // threads per line.  Measured on gfx950 (Float64 passes of
+            // 256^3, 512x512x64, 1024x256x128, 64x64x1024): faster for REDFT10
+            // from n = 256 (strided 256-point pass 0.148 -> 0.102 ms) and
+            // for REDFT01 from n = 512; slower below (128^3 REDFT10 passes
+            // 10.4-10.7 -> 11.6-11.9 us: the lines' threads no longer hide
+            // the latency), so shorter lines keep the n-point FFT.
+            const bool half_ok = (p->flags & DFFT_FLAG_DCT_HALF_LENGTH) ||
+                                 (!(p->flags & DFFT_FLAG_DCT_FULL_LENGTH) && nl >= (op == OP_DCT2 ? 256 : 512));
+            if ((op == OP_DCT2 || op == OP_DCT3) && nl % 4 == 0 && half_ok) {
+                AxisPass &a = p->passes.back().ap;
+                const int hop = op == OP_DCT2 ? OP_DCT2H : OP_DCT3H;
+                std::vector<int> R;
+                int tpl = 0;
+                if (fixed_schedule(p->prec, a.contig, hop == OP_DCT2H ? 3 : 2, hop, nl / 2, R, tpl)) {
+                    a.op = hop;
+                    a.sp.L = nl / 2;
+                }
+            }
         }
     }
 

@@ -25,7 +25,10 @@ enum PassOp : int {
     OP_DST1 = 11,     // RODFT00: L = 2(n+1) FFT of the odd extension, -Im of bins 1..n
     OP_DCT4 = 12,     // REDFT11: L = 2n FFT of x_j e^{-i pi j/2n}; 2 Re(e^{-i pi (2k+1)/4n} F_k)
     OP_DST4 = 13,     // RODFT11: same FFT; -2 Im(e^{-i pi (2k+1)/4n} F_k)
-    OP_COUNT = 14
+    OP_COUNT = 14,    // ops of the runtime engine (k_axis); the two below are catalog-only
+    // REDFT10 / REDFT01 of n = 2H reals (H even) through an H-point complex FFT
+    OP_DCT2H = 14,    // FFT of z_m = v_2m + i v_2m+1 (Makhoul's v), then R2C and DCT-II post-twiddles
+    OP_DCT3H = 15     // DCT-III and C2R pre-twiddles, H-point inverse FFT, z unpacked into the line
 };
 
 // Stockham schedule of one line length.

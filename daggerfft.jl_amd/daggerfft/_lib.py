@@ -39,6 +39,8 @@ FLAG_PLANE_PIPELINE = 0x40
 FLAG_BATCH_GROUPED = 0x80
 FLAG_BATCH_LANES = 0x100
 FLAG_MEASURE = 0x200
+FLAG_DCT_FULL_LENGTH = 0x400
+FLAG_DCT_HALF_LENGTH = 0x800
 
 _lib = None
 _lock = threading.Lock()

@@ -102,6 +102,11 @@ extern "C" {
                                          * creation (FFTW.MEASURE, src/fft.jl:127), and a batch's
                                          * launch schedule; synchronises the device and saves the
                                          * cache if dfft_tune_autosave is set */
+#define DFFT_FLAG_DCT_FULL_LENGTH 0x400u /* REDFT10/REDFT01 always through the n-point complex FFT */
+#define DFFT_FLAG_DCT_HALF_LENGTH 0x800u /* REDFT10/REDFT01 of n = 4k through the n/2-point complex
+                                          * FFT of packed pairs wherever a kernel exists (n = 16..1024);
+                                          * by default only from n = 256 (REDFT10) / 512 (REDFT01),
+                                          * where it measured faster.  With both flags: full length */
 
 typedef struct dfft_plan dfft_plan;
 

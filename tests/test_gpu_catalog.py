@@ -190,10 +190,11 @@ def test_catalog_pow2_alternative_radix_lists(gpu, tmp_path, prec):
 @pytest.mark.parametrize("variant", [2])
 @pytest.mark.parametrize("prec", [1, 0])
 def test_catalog_dct_pow2(gpu, prec, variant, tmp_path):
-    """DCT-II / DCT-III catalog kernels (split 16-value schedule, forced
-    through the schedule cache) of the powers of two 16..1024, along the
-    contiguous axis (paired loads and stores) and a strided axis, on 2 and 6
-    lines, against the oracle."""
+    """DCT-II / DCT-III catalog kernels through the n-point FFT
+    (DFFT_FLAG_DCT_FULL_LENGTH; split 16-value schedule, forced through the
+    schedule cache) of the powers of two 16..1024, along the contiguous axis
+    (paired loads and stores) and a strided axis, on 2 and 6 lines, against
+    the oracle."""
     import daggerfft as D
     from daggerfft import _lib as L
     dt = np.float64 if prec else np.float32
@@ -214,7 +215,8 @@ def test_catalog_dct_pow2(gpu, prec, variant, tmp_path):
         for n, shape, ax, layout, sc in cases:
             a = O.fill(n + ax, shape, dt)
             for kind, okind in ((D.REDFT10, O.REDFT10), (D.REDFT01, O.REDFT01)):
-                p = D.Plan(shape, L.R2R, -1, prec, D.Pencil(), (ax + 1,), r2r_kinds=(kind,))
+                p = D.Plan(shape, L.R2R, -1, prec, D.Pencil(), (ax + 1,), r2r_kinds=(kind,),
+                           flags=L.FLAG_DCT_FULL_LENGTH)
                 q = p.passes()[0]
                 assert q["variant"] == variant and q["op"] in ("DCT2", "DCT3"), q
                 G = p.execute(D.DeviceArray.from_host(a), D.DeviceArray(shape, dt))
@@ -222,6 +224,37 @@ def test_catalog_dct_pow2(gpu, prec, variant, tmp_path):
                 assert e <= tol, (n, shape, ax, kind, e)
     finally:
         D.tune_clear()
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+def test_catalog_dct_half_length(gpu, prec):
+    """REDFT10 / REDFT01 of n = 16..1024 reals through the n/2-point complex
+    FFT (DFFT_FLAG_DCT_HALF_LENGTH): along the contiguous axis (quad loads
+    and stores across mirror lanes) and the strided axes, on 2 and 6 lines,
+    against the oracle; and equal, to rounding, to the n-point path.  Without
+    a flag the plan takes the half-length path from n = 256 (REDFT10) and
+    n = 512 (REDFT01)."""
+    import daggerfft as D
+    from daggerfft import _lib as L
+    dt = np.float64 if prec else np.float32
+    tol = 1e-12 if prec else 1e-5
+    for n in (16, 32, 64, 128, 256, 512, 1024):
+        for shape, ax in (((n, 3, 2), 0), ((n, 2, 1), 0), ((2, n, 3), 1), ((3, 2, n), 2)):
+            a = O.fill(3 * n + ax, shape, dt)
+            for kind, okind, op in ((D.REDFT10, O.REDFT10, "DCT2H"), (D.REDFT01, O.REDFT01, "DCT3H")):
+                p = D.Plan(shape, L.R2R, -1, prec, D.Pencil(), (ax + 1,), r2r_kinds=(kind,),
+                           flags=L.FLAG_DCT_HALF_LENGTH)
+                q = p.passes()[0]
+                assert q["op"] == op and q["L"] == n // 2 and q["nlog"] == n, q
+                d = D.Plan(shape, L.R2R, -1, prec, D.Pencil(), (ax + 1,), r2r_kinds=(kind,)).passes()[0]
+                assert d["op"] == (op if n >= (256 if op == "DCT2H" else 512) else op[:4]), d
+                G = p.execute(D.DeviceArray.from_host(a), D.DeviceArray(shape, dt)).to_host()
+                e = O.rel_l2(G.astype(np.float64), O.r2rn(a.astype(np.float64), okind, (ax + 1,)))
+                assert e <= tol, (n, shape, ax, op, e)
+                f = D.Plan(shape, L.R2R, -1, prec, D.Pencil(), (ax + 1,), r2r_kinds=(kind,),
+                           flags=L.FLAG_DCT_FULL_LENGTH)
+                F = f.execute(D.DeviceArray.from_host(a), D.DeviceArray(shape, dt)).to_host()
+                assert O.rel_l2(G.astype(np.float64), F.astype(np.float64)) <= tol
 
 
 @pytest.mark.parametrize("swz", [97, 1021])
