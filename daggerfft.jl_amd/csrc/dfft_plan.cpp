@@ -469,8 +469,9 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
         // 128-thread workgroups by default (64 for the split schedule): many
         // small tiles in flight per CU
         const int wg = contig_threads > 256 ? contig_threads : a.variant == 2 ? 64 : 128;
-        int tc = (tune && tune->tc > 0) ? tune->tc : std::max(1, wg / tpl);
-        if (a.blue && tpl < 512) {
+        int tc = (tune && tune->tc > 0) ? tune->tc : (conv_tune && conv_tune->tc > 0) ? conv_tune->tc
+                                                                                     : std::max(1, wg / tpl);
+        if (a.blue && tpl < 512 && !(conv_tune && conv_tune->tc > 0)) {
             // Bluestein lines have odd thread counts (105 for M = 525): take
             // the line count in 128..512 threads that leaves the fewest idle
             // lanes in the last wavefront

@@ -209,6 +209,25 @@ static std::vector<std::vector<int>> radix_candidates(int L) {
 static std::vector<Tuning> candidates(const PassPlan &pp, int level, int max_threads) {
     const AxisPass &a = pp.ap;
     std::vector<Tuning> out;
+    if (a.blue) {
+        // a convolution pass (Rader, Bluestein) keeps its engine and takes
+        // only its tile width and swizzle from the cache
+        const int var = a.blue == 2 ? 2 : a.variant ? 3 : 0;
+        const std::vector<int> tcs = a.contig ? std::vector<int>{2, 4, 6, 8, 12, 16, 24, 32}
+                                              : std::vector<int>{8, 16, 32, 64};
+        const std::vector<int> swizzles = a.contig ? std::vector<int>{1} : std::vector<int>{1, 97, 1021};
+        for (int tc : tcs) {
+            if (tc * a.TPL > max_threads) continue;
+            for (int sw : swizzles) {
+                Tuning t;
+                t.variant = var;
+                t.tc = tc;
+                t.swz = sw;
+                out.push_back(t);
+            }
+        }
+        return out;
+    }
     std::vector<int> tcs = a.contig ? std::vector<int>{1, 2, 4, 8, 16} : std::vector<int>{4, 8, 16, 32, 64};
     std::vector<int> pads = a.contig ? (level >= 2 ? std::vector<int>{3, 4, 5} : std::vector<int>{3, 4})
                                      : std::vector<int>{0};

@@ -6,7 +6,8 @@ saves the fastest schedule of every axis pass.
   python daggerfft.jl_amd/wisdom/make_wisdom.py [out.txt] [--quick]
 
 Shapes: 512^3 ComplexF64 (full search), 128^3 ComplexF64, 256^3 R2C and C2R Float64,
-384x240x210 ComplexF64, the DCT-II/III configurations, and every distinct
+384x240x210 ComplexF64, the DCT-II/III configurations, the prime-length
+(Rader and Bluestein) shapes of the bench, and every distinct
 request shape of the mixed serving trace (cubic 64..512, C2C and R2C, both
 precisions); then the launch schedule of the 64-transform mixed batch
 (grouped against lanes, with the lanes' passes tuned).  Progress is printed
@@ -34,7 +35,11 @@ def main():
             ((256, 256, 256), L.R2C, L.F64, None, 1), ((256, 256, 256), L.C2R, L.F64, None, 1),
             ((384, 240, 210), L.C2C, L.F64, None, 1),
             ((256, 256), L.R2R, L.F64, (D.REDFT10, D.REDFT10), 1),
-            ((128, 128, 128), L.R2R, L.F64, (D.REDFT10,) * 3, 1)]
+            ((128, 128, 128), L.R2R, L.F64, (D.REDFT10,) * 3, 1),
+            # convolution passes (Rader, Bluestein): tile width and swizzle
+            ((257, 257, 257), L.C2C, L.F64, None, 1), ((251, 251, 251), L.C2C, L.F64, None, 1),
+            ((257, 257, 257), L.C2C, L.F32, None, 1), ((131, 131, 131), L.C2C, L.F64, None, 1),
+            ((127, 127, 127), L.C2C, L.F64, None, 1), ((1009, 128, 128), L.C2C, L.F64, None, 1)]
     seen = set()
     for q in S.serving_trace():
         key = (q.shape, q.kind, q.precision)

@@ -9,6 +9,8 @@ it -- on the runtime engine (M 7-smooth).
 Parity with the oracle (whose recursive FFT ends in direct DFTs of the prime
 factors, pinned against scipy in test_oracle.py): ComplexF64/Float64 rel-L2
 <= 1e-12; ComplexF32/Float32 against the FP64 oracle <= 2e-5."""
+import os
+
 import numpy as np
 import pytest
 
@@ -252,18 +254,18 @@ def test_lengths_beyond_one_tile_are_refused(gpu):
     assert ei.value.status == L.DFFT_ERR_UNSUPPORTED_SIZE
 
 
-def test_autotune_skips_bluestein_passes(gpu):
-    """The autotuner has no candidates for a convolution pass (Bluestein or
-    Rader): tuning a shape with prime axes records entries only for its
-    smooth axes, and the plan afterwards still runs convolution passes and
-    matches the oracle."""
+def test_autotune_covers_convolution_passes(gpu):
+    """The autotuner times a convolution pass (Bluestein or Rader) over tile
+    widths only, keeping its engine: tuning a shape with prime axes records
+    an entry per axis, and the plan afterwards still runs convolution passes
+    and matches the oracle."""
     import daggerfft as D
     from daggerfft import _lib as L
     shape = (11, 64, 13)
     try:
         D.tune_clear()
         D.autotune(shape, L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3), level=1)
-        assert D.tune_entries() == 1  # the 64-point axis only
+        assert D.tune_entries() == 3  # the 64-point axis and both convolution passes
         p = D.Plan(shape, L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3))
         assert [q["bluestein"] > 0 or q["rader"] > 0 for q in p.passes() if q["axis"] != 1] == [True, True]
         a = O.fill(27, shape, np.complex128)
@@ -309,3 +311,40 @@ def test_longest_real_bluestein_lines(gpu, n, prec):
     assert O.rel_l2(F.to_host().astype(np.complex128), ref) <= tol
     y = D.irfft(_dev(ref.astype(cdt)), n, decomp=D.Pencil(), dims=(1, 2, 3))
     assert O.rel_l2(y.to_host().astype(np.float64), x.astype(np.float64)) <= tol
+
+
+def test_convolution_pass_tile_width_from_cache_and_tuner(gpu, tmp_path):
+    """A convolution pass takes its tile width from the schedule cache along
+    the contiguous axis too (Bluestein 131, Rader 257), gives the oracle's
+    result with it, and the autotuner times tile widths of convolution
+    passes (its entries are then in the cache)."""
+    import math
+
+    import daggerfft as D
+    from daggerfft import _lib as L
+    shipped = os.path.join(os.path.dirname(D.__file__), "..", "wisdom", "gfx950.txt")
+    try:
+        for n, var in ((131, 3), (257, 2)):
+            shape = (n, 6, 4)
+            sc = int(math.floor(math.log2(n * 24)))
+            path = str(tmp_path / ("tc%d.txt" % n))
+            with open(path, "w") as f:
+                f.write("1/0/%d/1/%d - 0 4 0 %d 1\n" % (n, sc, var))
+            D.tune_clear()
+            D.tune_load(path)
+            p = D.Plan(shape, L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3))
+            q = [x for x in p.passes() if x["axis"] == 0][0]
+            assert q["TC"] == 4 and (q["rader"] if var == 2 else q["bluestein"]) > 0, q
+            a = O.fill(40 + n, shape, np.complex128)
+            F = p.execute(D.DeviceArray.from_host(a), D.DeviceArray(shape, np.complex128))
+            assert O.rel_l2(F.to_host(), O.fftn(a)) <= 1e-12
+        D.tune_clear()
+        D.autotune((131, 8, 8), kind=L.C2C, sign=-1, precision=L.F64, level=1)
+        assert D.tune_entries() >= 3
+        p = D.Plan((131, 8, 8), L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3))
+        a = O.fill(41, (131, 8, 8), np.complex128)
+        F = p.execute(D.DeviceArray.from_host(a), D.DeviceArray((131, 8, 8), np.complex128))
+        assert O.rel_l2(F.to_host(), O.fftn(a)) <= 1e-12
+    finally:
+        D.tune_clear()
+        D.tune_load(shipped)
