@@ -749,9 +749,12 @@ int build_plan(dfft_plan *p) {
             // from n = 256 (strided 256-point pass 0.148 -> 0.102 ms) and
             // for REDFT01 from n = 512; slower below (128^3 REDFT10 passes
             // 10.4-10.7 -> 11.6-11.9 us: the lines' threads no longer hide
-            // the latency), so shorter lines keep the n-point FFT.
+            // the latency), so shorter lines keep the n-point FFT, and so do
+            // arrays under 4M elements (256^2 REDFT10: 16 workgroups per
+            // launch, 7.0 -> 8.3 us).
             const bool half_ok = (p->flags & DFFT_FLAG_DCT_HALF_LENGTH) ||
-                                 (!(p->flags & DFFT_FLAG_DCT_FULL_LENGTH) && nl >= (op == OP_DCT2 ? 256 : 512));
+                                 (!(p->flags & DFFT_FLAG_DCT_FULL_LENGTH) && nl >= (op == OP_DCT2 ? 256 : 512) &&
+                                  N >= (1LL << 22));
             if ((op == OP_DCT2 || op == OP_DCT3) && nl % 4 == 0 && half_ok) {
                 AxisPass &a = p->passes.back().ap;
                 const int hop = op == OP_DCT2 ? OP_DCT2H : OP_DCT3H;

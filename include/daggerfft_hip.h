@@ -105,8 +105,9 @@ extern "C" {
 #define DFFT_FLAG_DCT_FULL_LENGTH 0x400u /* REDFT10/REDFT01 always through the n-point complex FFT */
 #define DFFT_FLAG_DCT_HALF_LENGTH 0x800u /* REDFT10/REDFT01 of n = 4k through the n/2-point complex
                                           * FFT of packed pairs wherever a kernel exists (n = 16..1024);
-                                          * by default only from n = 256 (REDFT10) / 512 (REDFT01),
-                                          * where it measured faster.  With both flags: full length */
+                                          * by default only from n = 256 (REDFT10) / 512 (REDFT01) in
+                                          * arrays of 4M elements or more, where it measured faster.
+                                          * With both flags: full length */
 
 typedef struct dfft_plan dfft_plan;
 

@@ -232,8 +232,8 @@ def test_catalog_dct_half_length(gpu, prec):
     FFT (DFFT_FLAG_DCT_HALF_LENGTH): along the contiguous axis (quad loads
     and stores across mirror lanes) and the strided axes, on 2 and 6 lines,
     against the oracle; and equal, to rounding, to the n-point path.  Without
-    a flag the plan takes the half-length path from n = 256 (REDFT10) and
-    n = 512 (REDFT01)."""
+    a flag a plan takes the half-length path from n = 256 (REDFT10) and
+    n = 512 (REDFT01) in arrays of 4M elements or more."""
     import daggerfft as D
     from daggerfft import _lib as L
     dt = np.float64 if prec else np.float32
@@ -247,7 +247,7 @@ def test_catalog_dct_half_length(gpu, prec):
                 q = p.passes()[0]
                 assert q["op"] == op and q["L"] == n // 2 and q["nlog"] == n, q
                 d = D.Plan(shape, L.R2R, -1, prec, D.Pencil(), (ax + 1,), r2r_kinds=(kind,)).passes()[0]
-                assert d["op"] == (op if n >= (256 if op == "DCT2H" else 512) else op[:4]), d
+                assert d["op"] == op[:4], d  # small arrays keep the n-point FFT
                 G = p.execute(D.DeviceArray.from_host(a), D.DeviceArray(shape, dt)).to_host()
                 e = O.rel_l2(G.astype(np.float64), O.r2rn(a.astype(np.float64), okind, (ax + 1,)))
                 assert e <= tol, (n, shape, ax, op, e)
@@ -255,6 +255,11 @@ def test_catalog_dct_half_length(gpu, prec):
                            flags=L.FLAG_DCT_FULL_LENGTH)
                 F = f.execute(D.DeviceArray.from_host(a), D.DeviceArray(shape, dt)).to_host()
                 assert O.rel_l2(G.astype(np.float64), F.astype(np.float64)) <= tol
+    for shape, ax, ok2, ok3 in (((256, 256, 64), 0, True, False), ((512, 128, 64), 0, True, True),
+                                ((128, 256, 128), 0, False, False), ((256, 256, 32), 0, False, False)):
+        for kind, op, ok in ((D.REDFT10, "DCT2", ok2), (D.REDFT01, "DCT3", ok3)):
+            d = D.Plan(shape, L.R2R, -1, prec, D.Pencil(), (ax + 1,), r2r_kinds=(kind,)).passes()[0]
+            assert d["op"] == (op + "H" if ok else op), (shape, d)
 
 
 @pytest.mark.parametrize("swz", [97, 1021])
