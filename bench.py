@@ -281,6 +281,7 @@ def configs():
     outs = [x.similar() for x in ins]
     res = {}
     for name, flags in (("grouped", L.FLAG_BATCH_GROUPED), ("grouped_graph", L.FLAG_BATCH_GROUPED | L.FLAG_GRAPH),
+                        ("persistent", L.FLAG_BATCH_PERSISTENT),
                         ("lanes", L.FLAG_BATCH_LANES), ("lanes_graph", L.FLAG_BATCH_LANES | L.FLAG_GRAPH)):
         b = Batch(descs, flags=flags)
         ms = _time_events(lambda: b.execute(ins, outs), 10)
@@ -293,7 +294,7 @@ def configs():
         if prec == L.F32:  # FP32 result against the FP64 transform of the same (widened) input
             x64 = D.DeviceArray(shape, np.complex128, tensor=x.tensor.to(torch.complex128))
             worst32 = max(worst32, rel(y.tensor.to(torch.complex128), D.fft(x64).tensor))
-    out["batch64_mixed_c2c"] = {"transforms": 64, "default_schedule": "grouped" if Batch(descs).launches == 3 else "lanes",
+    out["batch64_mixed_c2c"] = {"transforms": 64, "default_schedule": {1: "persistent", 3: "grouped"}.get(Batch(descs).launches, "lanes"),
                                 **res, "sequential_plans_ms": round(seq, 4),
                                 "f32_rel_l2_vs_f64_max": worst32, "algorithmic_bytes": b.algorithmic_bytes()}
     del ins, outs

@@ -15,6 +15,7 @@
 //     pass of chunk i-1, which the previous launch just wrote, so that chunk
 //     is read back while it is still in the 256 MB memory-side Infinity
 //     Cache.  The dims[2] pass follows as one ordinary launch.
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -124,18 +125,27 @@ int gfix_lookup(int prec, int contig, int L) {
 
 const GFixEntry &gfix_entry(int id) { return g_gfix[id]; }
 
-// FIX: the launch holds compile-time schedules (GFIX_THREADS threads only);
-// its register budget is 12 waves per CU (168 VGPRs): the 16-value schedules
-// spill at 128.  Launches of runtime-engine items only take the kernel
-// without them, whose register count follows from the runtime engine alone.
-template <int THREADS, bool FIX>
-__global__ __launch_bounds__(THREADS, FIX ? 3 : 1) void k_group(GroupArgs g) {
-    extern __shared__ __align__(16) unsigned char dfft_smem[];
-    const int item = __builtin_amdgcn_readfirstlane(g.block_item[blockIdx.x]);
-    const GroupItem &it = g.items[item];
-    const void *in = reinterpret_cast<const unsigned char *>(g.slots[it.in_slot]) + it.in_off;
-    void *out = reinterpret_cast<unsigned char *>(g.slots[it.out_slot]) + it.out_off;
-    const int64_t tile = (int64_t)blockIdx.x - it.first_block;
+// A pointer the whole wavefront shares, held in scalar registers (the item
+// table is read through vector loads, so the compiler cannot tell)
+template <class P>
+__device__ __forceinline__ P *uniform_ptr(P *p) {
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return reinterpret_cast<P *>(lo | (hi << 32));
+}
+
+// One workgroup's tile of a grouped launch: block `blk` of the level whose
+// items and block -> item map are given; the slots come from the kernel
+// arguments.  FIX: the launch holds compile-time schedules.
+template <bool FIX, class Args>
+__device__ __forceinline__ void group_block(const Args &g, const GroupItem *items, const int *block_item, int64_t blk,
+                                            int tid, void *smem) {
+    const int item = __builtin_amdgcn_readfirstlane(block_item[blk]);
+    const GroupItem &it = items[item];
+    const void *in = uniform_ptr(reinterpret_cast<const unsigned char *>(g.slots[it.in_slot]) + it.in_off);
+    void *out = uniform_ptr(reinterpret_cast<unsigned char *>(g.slots[it.out_slot]) + it.out_off);
+    const int64_t tile = blk - it.first_block;
     if constexpr (FIX) {
         const int gfix = __builtin_amdgcn_readfirstlane(it.gfix);
         if (gfix >= 0) {
@@ -144,12 +154,65 @@ __global__ __launch_bounds__(THREADS, FIX ? 3 : 1) void k_group(GroupArgs g) {
             // reloaded after the tile's global stores, which the compiler
             // cannot tell apart from the item table
             const AxisPass a = it.ap;
-            gfix_tile(gfix, a, in, out, tile, (int)threadIdx.x, dfft_smem);
+            gfix_tile(gfix, a, in, out, tile, tid, smem);
             return;
         }
     }
-    if (it.prec) group_tile<double>(it.ap, in, out, tile, (int)threadIdx.x, dfft_smem);
-    else group_tile<float>(it.ap, in, out, tile, (int)threadIdx.x, dfft_smem);
+    if (it.prec) group_tile<double>(it.ap, in, out, tile, tid, smem);
+    else group_tile<float>(it.ap, in, out, tile, tid, smem);
+}
+
+// FIX: the launch holds compile-time schedules (GFIX_THREADS threads only);
+// its register budget is 12 waves per CU (168 VGPRs): the 16-value schedules
+// spill at 128.  Launches of runtime-engine items only take the kernel
+// without them, whose register count follows from the runtime engine alone.
+template <int THREADS, bool FIX>
+__global__ __launch_bounds__(THREADS, FIX ? 3 : 1) void k_group(GroupArgs g) {
+    extern __shared__ __align__(16) unsigned char dfft_smem[];
+    group_block<FIX>(g, g.items, g.block_item, (int64_t)blockIdx.x, (int)threadIdx.x, dfft_smem);
+}
+
+// The whole group as one cooperative launch: every workgroup of the grid is
+// resident at once (hipLaunchCooperativeKernel refuses a grid larger than the
+// device holds), walks the tiles of level l with a stride of the grid size,
+// and meets the others at a grid barrier before level l + 1 reads what level
+// l wrote.  Every workgroup runs every level's loop, so each reaches each of
+// the nlevels - 1 barriers; no workgroup waits on a particular other one.
+// (Tiles dealt from per-level ticket counters instead measured within 2%,
+// and counters would make two launches of one group on different streams
+// race.)
+constexpr int PERSIST_MAX_LEVELS = 4;
+struct PersistArgs {
+    const GroupItem *items;  // every item, level after level
+    const int *block_item;   // every level's block -> item map, concatenated
+    int nlevels;
+    int first_item[PERSIST_MAX_LEVELS];
+    int64_t map_off[PERSIST_MAX_LEVELS];
+    int64_t nblocks[PERSIST_MAX_LEVELS];
+    void *slots[GROUP_MAX_SLOTS];
+};
+
+__global__ __launch_bounds__(GFIX_THREADS, 3) void k_group_persist(PersistArgs g) {
+    extern __shared__ __align__(16) unsigned char dfft_smem[];
+    for (int l = 0; l < g.nlevels; ++l) {
+        const GroupItem *items = g.items + g.first_item[l];
+        const int *map = g.block_item + g.map_off[l];
+        const int nb = (int)g.nblocks[l];  // <= 0x7fffffff - grid (group_persistent_prepare)
+        for (int b = (int)blockIdx.x; b < nb; b += (int)gridDim.x) {
+            // the thread index passes an empty asm statement every tile, so
+            // the compiler cannot hoist the tiles' per-thread index
+            // arithmetic out of the loop (held across it, the 56 schedules'
+            // values spill)
+            int tid = (int)threadIdx.x;
+            asm volatile("" : "+v"(tid));
+            group_block<true>(g, items, map, b, tid, dfft_smem);
+            __syncthreads();  // the next tile reuses the LDS
+        }
+        // the grid barrier orders memory too: after a workgroup barrier its
+        // arriving workitem writes the L2 back, and invalidates it once all
+        // workgroups have arrived
+        if (l + 1 < g.nlevels) cooperative_groups::this_grid().sync();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -169,6 +232,8 @@ struct GroupExec {
     std::vector<GroupLaunch> launches;
     GroupItem *d_items = nullptr;
     int *d_map = nullptr;
+    int persist_grid = 0;  // group_persistent_prepare: workgroups of the single launch
+    size_t persist_lds = 0;
 };
 
 void group_destroy(GroupExec *g) {
@@ -268,6 +333,66 @@ int group_launch(const GroupExec *g, void *const *slots, int nslots, hipStream_t
             hipLaunchKernelGGL((k_group<256, false>), dim3((unsigned)L.nblocks), dim3(256), L.lds, st, a);
         if (hipError_t e = hipGetLastError()) return hip_fail(e, "grouped launch");
     }
+    return DFFT_OK;
+}
+
+int group_persistent_prepare(GroupExec *g) {
+    if (g->launches.empty() || g->launches.size() > (size_t)PERSIST_MAX_LEVELS) return DFFT_ERR_UNSUPPORTED;
+    size_t lds = 0;
+    int64_t most = 0;
+    for (const auto &L : g->launches) {
+        // one kernel for every level: the 256-thread tiles only
+        if (L.threads != GFIX_THREADS) return DFFT_ERR_UNSUPPORTED_SIZE;
+        lds = std::max(lds, L.lds);
+        most = std::max(most, L.nblocks);
+    }
+    int dev = 0, coop = 0, cus = 0, per = 0;
+    if (hipError_t e = hipGetDevice(&dev)) return hip_fail(e, "hipGetDevice");
+    if (hipError_t e = hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev))
+        return hip_fail(e, "hipDeviceGetAttribute");
+    if (!coop) return DFFT_ERR_UNSUPPORTED;
+    if (hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev))
+        return hip_fail(e, "hipDeviceGetAttribute");
+    if (lds > 65536) {
+        if (hipError_t e = hipFuncSetAttribute((const void *)k_group_persist,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, axis_lds_limit()))
+            return hip_fail(e, "hipFuncSetAttribute");
+    }
+    if (hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_group_persist, GFIX_THREADS, lds))
+        return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    if (per < 1 || cus < 1) return DFFT_ERR_UNSUPPORTED_SIZE;
+    g->persist_grid = (int)std::min<int64_t>((int64_t)per * cus, most);
+    // the kernel's tile counter (int) steps by the grid size past the level's count
+    if (most > (int64_t)0x7fffffff - g->persist_grid) return DFFT_ERR_UNSUPPORTED_SIZE;
+    if (getenv("DFFT_TUNE_VERBOSE"))
+        fprintf(stderr, "persistent group: %d workgroups (%d per CU x %d CUs), %zu B LDS, levels of", g->persist_grid,
+                per, cus, lds);
+    if (getenv("DFFT_TUNE_VERBOSE")) {
+        for (const auto &L : g->launches) fprintf(stderr, " %lld", (long long)L.nblocks);
+        fprintf(stderr, " tiles\n");
+    }
+    g->persist_lds = lds;
+    return DFFT_OK;
+}
+
+int group_persistent_grid(const GroupExec *g) { return g->persist_grid; }
+
+int group_launch_persistent(const GroupExec *g, void *const *slots, int nslots, hipStream_t st) {
+    if (nslots > GROUP_MAX_SLOTS || g->persist_grid < 1) return DFFT_ERR_ARG;
+    PersistArgs a{};
+    a.items = g->d_items;
+    a.block_item = g->d_map;
+    a.nlevels = (int)g->launches.size();
+    for (int l = 0; l < a.nlevels; ++l) {
+        a.first_item[l] = g->launches[l].first_item;
+        a.map_off[l] = g->launches[l].map_off;
+        a.nblocks[l] = g->launches[l].nblocks;
+    }
+    for (int i = 0; i < nslots; ++i) a.slots[i] = slots[i];
+    void *kargs[] = {&a};
+    if (hipError_t e = hipLaunchCooperativeKernel((const void *)k_group_persist, dim3((unsigned)g->persist_grid),
+                                                  dim3(GFIX_THREADS), kargs, (unsigned)g->persist_lds, st))
+        return hip_fail(e, "cooperative launch");
     return DFFT_OK;
 }
 
@@ -374,6 +499,7 @@ struct dfft_batch {
     std::vector<dfft_batch_desc> descs;
     GroupExec *g = nullptr;
     GroupGraph *graph = nullptr;  // DFFT_FLAG_GRAPH: the launches replayed as one hipGraph
+    bool persistent = false;      // DFFT_FLAG_BATCH_PERSISTENT: the launches as one cooperative launch
     void *d_tw64 = nullptr, *d_tw32 = nullptr;
     size_t tw64_bytes = 0, tw32_bytes = 0;
     double alg_bytes = 0;
@@ -598,16 +724,18 @@ static std::string batch_key(int count, const dfft_batch_desc *descs) {
 }
 
 // Launch-schedule values of a batch's cache entry (Tuning::variant)
-constexpr int BATCH_PICK_GROUPED = 1, BATCH_PICK_LANES = 2;
+constexpr int BATCH_PICK_GROUPED = 1, BATCH_PICK_LANES = 2, BATCH_PICK_PERSISTENT = 3;
 
 extern "C" int dfft_batch_create(dfft_batch **batch, int count, const dfft_batch_desc *descs, uint32_t flags);
 
-// DFFT_FLAG_MEASURE: times the grouped schedule (when every pass fits it) and
-// the lanes schedule (each transform's own plan, its passes autotuned) on
-// scratch arrays of the batch's sizes, best of three, and returns the faster
+// DFFT_FLAG_MEASURE: times the grouped schedule (when every pass fits it), its
+// single cooperative launch (when every level fits 256-thread tiles) and the
+// lanes schedule (each transform's own plan, its passes autotuned) on scratch
+// arrays of the batch's sizes, best of three, and returns the fastest
 // (BATCH_PICK_*), or a negative DFFT_ERR_* code.
 static int measure_batch(int count, const dfft_batch_desc *descs, uint32_t flags) {
-    const uint32_t base = flags & ~(uint32_t)(DFFT_FLAG_GRAPH | DFFT_FLAG_BATCH_GROUPED | DFFT_FLAG_BATCH_LANES);
+    const uint32_t base = flags & ~(uint32_t)(DFFT_FLAG_GRAPH | DFFT_FLAG_BATCH_GROUPED | DFFT_FLAG_BATCH_LANES |
+                                              DFFT_FLAG_BATCH_PERSISTENT);
     std::vector<void *> in(count, nullptr), out(count, nullptr);
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -628,13 +756,15 @@ static int measure_batch(int count, const dfft_batch_desc *descs, uint32_t flags
         else if (hipError_t e3 = hipEventCreate(&e1)) rc = hip_fail(e3, "hipEventCreate");
     }
     if (rc == DFFT_OK && hipDeviceSynchronize() != hipSuccess) rc = hip_fail(hipGetLastError(), "fill");
-    float best[2] = {-1.0f, -1.0f};
-    for (int k = 0; k < 2 && rc == DFFT_OK; ++k) {
+    // candidates: grouped, persistent, lanes (BATCH_PICK_* of kind[k])
+    const uint32_t kflag[3] = {DFFT_FLAG_BATCH_GROUPED, DFFT_FLAG_BATCH_PERSISTENT, DFFT_FLAG_BATCH_LANES};
+    const int kind[3] = {BATCH_PICK_GROUPED, BATCH_PICK_PERSISTENT, BATCH_PICK_LANES};
+    float best[3] = {-1.0f, -1.0f, -1.0f};
+    for (int k = 0; k < 3 && rc == DFFT_OK; ++k) {
         dfft_batch *b = nullptr;
-        const uint32_t sf = base | (k == 0 ? DFFT_FLAG_BATCH_GROUPED : DFFT_FLAG_BATCH_LANES);
-        const int crc = dfft_batch_create(&b, count, descs, sf);
+        const int crc = dfft_batch_create(&b, count, descs, base | kflag[k]);
         if (crc != DFFT_OK) {
-            if (k == 1) rc = crc;  // lanes always builds; a grouped batch may not
+            if (k == 2) rc = crc;  // lanes always builds; a grouped or persistent batch may not
             continue;
         }
         bool ok = dfft_batch_execute(b, in.data(), out.data(), st) == DFFT_OK;  // warm-up
@@ -659,8 +789,12 @@ static int measure_batch(int count, const dfft_batch_desc *descs, uint32_t flags
     }
     if (rc != DFFT_OK) return -rc;
     if (getenv("DFFT_TUNE_VERBOSE"))
-        fprintf(stderr, "tune batch of %d: grouped %.4f ms, lanes %.4f ms\n", count, best[0], best[1]);
-    return (best[0] >= 0 && best[0] <= best[1]) ? BATCH_PICK_GROUPED : BATCH_PICK_LANES;
+        fprintf(stderr, "tune batch of %d: grouped %.4f ms, persistent %.4f ms, lanes %.4f ms\n", count, best[0],
+                best[1], best[2]);
+    int pick = 2;  // lanes, which always built
+    for (int k = 0; k < 2; ++k)
+        if (best[k] >= 0 && best[k] <= best[pick]) pick = k;
+    return kind[pick];
 }
 
 extern "C" {
@@ -687,7 +821,15 @@ int dfft_batch_create(dfft_batch **batch, int count, const dfft_batch_desc *desc
         batch_free(b);
         return DFFT_ERR_ARG;
     }
-    bool grouped = (flags & DFFT_FLAG_BATCH_GROUPED) != 0;
+    // the persistent schedule is the grouped one in a single launch: nothing
+    // left for a graph to chain, and no lanes
+    if ((flags & DFFT_FLAG_BATCH_PERSISTENT) && (flags & (DFFT_FLAG_BATCH_LANES | DFFT_FLAG_GRAPH))) {
+        batch_free(b);
+        return DFFT_ERR_ARG;
+    }
+    b->persistent = (flags & DFFT_FLAG_BATCH_PERSISTENT) != 0;
+    bool persist_cached = false;  // the single launch came from the cache, not a flag
+    bool grouped = (flags & (DFFT_FLAG_BATCH_GROUPED | DFFT_FLAG_BATCH_PERSISTENT)) != 0;
     if (!grouped && !(flags & DFFT_FLAG_BATCH_LANES)) {
         // default: the schedule cache's pick for this mix (DFFT_FLAG_MEASURE
         // times both schedules when there is none); without one, grouped
@@ -707,7 +849,10 @@ int dfft_batch_create(dfft_batch **batch, int count, const dfft_batch_desc *desc
             have = true;
         }
         if (have) {
-            grouped = pick.variant == BATCH_PICK_GROUPED;
+            grouped = pick.variant == BATCH_PICK_GROUPED || pick.variant == BATCH_PICK_PERSISTENT;
+            // a cached single launch only where no graph was asked for
+            b->persistent = pick.variant == BATCH_PICK_PERSISTENT && !(flags & DFFT_FLAG_GRAPH);
+            persist_cached = b->persistent;
         } else {
             grouped = true;
             for (int t = 0; t < count && grouped; ++t) {
@@ -824,6 +969,14 @@ int dfft_batch_create(dfft_batch **batch, int count, const dfft_batch_desc *desc
         }
     }
     rc = group_create(&b->g, levels);
+    if (rc == DFFT_OK && b->persistent) {
+        rc = group_persistent_prepare(b->g);
+        // a cached pick the device cannot launch falls back to the grouped launches
+        if (rc != DFFT_OK && persist_cached) {
+            b->persistent = false;
+            rc = DFFT_OK;
+        }
+    }
     if (rc == DFFT_OK && (flags & DFFT_FLAG_GRAPH) && !(b->graph = group_graph_new())) rc = DFFT_ERR_OOM;
     if (rc != DFFT_OK) {
         batch_free(b);
@@ -843,6 +996,7 @@ int dfft_batch_execute(dfft_batch *b, void *const *d_in, void *const *d_out, voi
         slots[2 * t + 1] = d_out[t];
     }
     if (b->lanes) return lanes_execute(b, d_in, d_out, (hipStream_t)stream);
+    if (b->persistent) return group_launch_persistent(b->g, slots.data(), (int)slots.size(), (hipStream_t)stream);
     if (b->graph) return group_launch_graph(b->g, b->graph, slots.data(), (int)slots.size(), (hipStream_t)stream);
     return group_launch(b->g, slots.data(), (int)slots.size(), (hipStream_t)stream);
 }
@@ -867,7 +1021,7 @@ int dfft_batch_num_launches(const dfft_batch *b, int *launches) {
         for (const dfft_plan *p : b->plans) n += plan_pass_count(p);
         *launches = n;
     } else {
-        *launches = group_num_launches(b->g);
+        *launches = b->persistent ? 1 : group_num_launches(b->g);
     }
     return DFFT_OK;
 }

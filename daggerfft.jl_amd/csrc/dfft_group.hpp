@@ -49,6 +49,12 @@ void group_graph_destroy(GroupGraph *gg);
 // the group's launches as one hipGraph replay (recorded on first use and
 // whenever the slot pointers change)
 int group_launch_graph(const GroupExec *g, GroupGraph *gg, void *const *slots, int nslots, hipStream_t st);
+// the whole group as one cooperative launch with a grid barrier between
+// levels: prepare once (every level must run 256-thread tiles; sizes the
+// grid to the workgroups the device holds at once), then launch
+int group_persistent_prepare(GroupExec *g);
+int group_persistent_grid(const GroupExec *g);
+int group_launch_persistent(const GroupExec *g, void *const *slots, int nslots, hipStream_t st);
 int group_num_launches(const GroupExec *g);
 int64_t group_blocks(const GroupExec *g);
 GroupItem make_group_item(const PassPlan &pp, int in_slot, int64_t in_off, int out_slot, int64_t out_off);

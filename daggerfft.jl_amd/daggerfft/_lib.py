@@ -41,6 +41,7 @@ FLAG_BATCH_LANES = 0x100
 FLAG_MEASURE = 0x200
 FLAG_DCT_FULL_LENGTH = 0x400
 FLAG_DCT_HALF_LENGTH = 0x800
+FLAG_BATCH_PERSISTENT = 0x1000
 
 _lib = None
 _lock = threading.Lock()

@@ -319,13 +319,17 @@ end
 
 # ---- batched transforms (dfft_batch_*; no reference counterpart) ----------
 # Many independent 3D transforms per call: one grouped launch per axis-pass
-# level (DFFT_FLAG_BATCH_GROUPED), or each transform's own plan on four
-# streams forked from the caller's (DFFT_FLAG_BATCH_LANES); by default the
-# library picks (grouped when every length has a compile-time grouped
-# schedule).  DFFT_FLAG_GRAPH replays the launches as one hipGraph.
+# level (DFFT_FLAG_BATCH_GROUPED), the same tiles in one cooperative launch
+# with a grid barrier between levels (DFFT_FLAG_BATCH_PERSISTENT), or each
+# transform's own plan on four streams forked from the caller's
+# (DFFT_FLAG_BATCH_LANES); by default the schedule cache's measured pick
+# (DFFT_FLAG_MEASURE times the three).  DFFT_FLAG_GRAPH replays the launches
+# as one hipGraph.
 const DFFT_FLAG_GRAPH = UInt32(0x10)
 const DFFT_FLAG_BATCH_GROUPED = UInt32(0x80)
 const DFFT_FLAG_BATCH_LANES = UInt32(0x100)
+const DFFT_FLAG_MEASURE = UInt32(0x200)
+const DFFT_FLAG_BATCH_PERSISTENT = UInt32(0x1000)
 
 struct BatchDesc            # dfft_batch_desc, field for field
     n::NTuple{3,Int64}

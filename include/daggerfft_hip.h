@@ -108,6 +108,8 @@ extern "C" {
                                           * by default only from n = 256 (REDFT10) / 512 (REDFT01) in
                                           * arrays of 4M elements or more, where it measured faster.
                                           * With both flags: full length */
+#define DFFT_FLAG_BATCH_PERSISTENT 0x1000u /* batch: the grouped schedule as one cooperative launch,
+                                            * a grid barrier between axis-pass levels */
 
 typedef struct dfft_plan dfft_plan;
 
@@ -285,14 +287,28 @@ int dfft_plan_export_wisdom(const dfft_plan *plan, const char *path);
  *   DFFT_FLAG_BATCH_LANES    each transform's own plan (the compile-time
  *                            kernels) on one of four streams forked from and
  *                            joined back into the caller's stream; results
- *                            equal each transform run alone bit for bit.
- * Neither flag: the schedule cache's pick for this mix of descriptors; with
- * DFFT_FLAG_MEASURE and no pick cached, both schedules are timed on scratch
- * arrays of the batch's sizes (the lanes' plans with their passes
- * autotuned) and the faster is cached (and saved with the cache when a save
- * path is set).  With no pick: grouped when every axis length of every
+ *                            equal each transform run alone bit for bit;
+ *   DFFT_FLAG_BATCH_PERSISTENT  the grouped schedule as ONE cooperative
+ *                            launch (hipLaunchCooperativeKernel): as many
+ *                            256-thread workgroups as the device holds at
+ *                            once loop over each level's tiles and meet at a
+ *                            grid barrier between levels.  Same tiles and
+ *                            results as DFFT_FLAG_BATCH_GROUPED, bit for bit.
+ *                            Every level must fit 256-thread tiles
+ *                            (DFFT_ERR_UNSUPPORTED_SIZE otherwise, e.g. a
+ *                            4096-point contiguous axis);
+ *                            DFFT_ERR_UNSUPPORTED on a device without
+ *                            cooperative launches.  With DFFT_FLAG_BATCH_LANES
+ *                            or DFFT_FLAG_GRAPH: DFFT_ERR_ARG.
+ * No schedule flag: the schedule cache's pick for this mix of descriptors;
+ * with DFFT_FLAG_MEASURE and no pick cached, the three schedules are timed on
+ * scratch arrays of the batch's sizes (the lanes' plans with their passes
+ * autotuned) and the fastest is cached (and saved with the cache when a save
+ * path is set).  A cached single-launch pick falls back to the grouped
+ * launches with DFFT_FLAG_GRAPH or on a device that cannot launch it.  With no pick: grouped when every axis length of every
  * transform is one of the compile-time grouped lengths and every transform
- * is C2C, else lanes.  Both flags: DFFT_ERR_ARG.
+ * is C2C, else lanes.  DFFT_FLAG_BATCH_GROUPED with DFFT_FLAG_BATCH_LANES:
+ * DFFT_ERR_ARG.
  * With DFFT_FLAG_GRAPH at creation the launches are recorded once as a
  * hipGraph (the lanes as parallel branches) and replayed by every execute,
  * re-recorded when the buffer pointers change. */

@@ -203,3 +203,18 @@ def test_planted_mistakes_are_caught(c_layout):
     want = [c_param_as_julia(p) for p in protos["dfft_server_wait"][1]]
     assert want == ["Ptr{Cvoid}", "Int64", "Cdouble"]
     assert ["Ptr{Cvoid}", "Cint", "Cdouble"] != want
+
+
+def _flag_values(text, pattern):
+    return {m.group(1): int(m.group(2), 16) for m in re.finditer(pattern, text)}
+
+
+def test_flag_constants_match_header():
+    """Every DFFT_* constant the shim defines as UInt32(0x...) has the value
+    the header #defines for the same name (plan and batch flags)."""
+    jl = _flag_values(open(JL).read(), r"const\s+(DFFT_\w+)\s*=\s*UInt32\(0x([0-9a-fA-F]+)\)")
+    hdr = _flag_values(open(HDR).read(), r"#define\s+(DFFT_\w+)\s+0x([0-9a-fA-F]+)u?")
+    assert "DFFT_FLAG_BATCH_PERSISTENT" in jl and "DFFT_FLAG_UNNORMALIZED" in hdr
+    for name, v in jl.items():
+        assert name in hdr, name
+        assert hdr[name] == v, (name, hex(v), hex(hdr[name]))
