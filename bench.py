@@ -280,8 +280,11 @@ def configs():
     ins = [D.rand(shape, np.complex128 if prec else np.complex64, seed=1000 + t) for t, (shape, prec) in enumerate(cfg)]
     outs = [x.similar() for x in ins]
     res = {}
+    # (the single cooperative launch is timed at the very end of the run,
+    # persistent_batch(): after a process's first cooperative launch the HIP
+    # runtime serialises work across streams, which would slow the lanes and
+    # the serving loop measured after it)
     for name, flags in (("grouped", L.FLAG_BATCH_GROUPED), ("grouped_graph", L.FLAG_BATCH_GROUPED | L.FLAG_GRAPH),
-                        ("persistent", L.FLAG_BATCH_PERSISTENT),
                         ("lanes", L.FLAG_BATCH_LANES), ("lanes_graph", L.FLAG_BATCH_LANES | L.FLAG_GRAPH)):
         b = Batch(descs, flags=flags)
         ms = _time_events(lambda: b.execute(ins, outs), 10)
@@ -294,7 +297,7 @@ def configs():
         if prec == L.F32:  # FP32 result against the FP64 transform of the same (widened) input
             x64 = D.DeviceArray(shape, np.complex128, tensor=x.tensor.to(torch.complex128))
             worst32 = max(worst32, rel(y.tensor.to(torch.complex128), D.fft(x64).tensor))
-    out["batch64_mixed_c2c"] = {"transforms": 64, "default_schedule": {1: "persistent", 3: "grouped"}.get(Batch(descs).launches, "lanes"),
+    out["batch64_mixed_c2c"] = {"transforms": 64, "default_schedule": "grouped" if Batch(descs).launches == 3 else "lanes",
                                 **res, "sequential_plans_ms": round(seq, 4),
                                 "f32_rel_l2_vs_f64_max": worst32, "algorithmic_bytes": b.algorithmic_bytes()}
     del ins, outs
@@ -438,6 +441,31 @@ def _smooth(m):
     return m == 1
 
 
+def persistent_batch():
+    """The 64-transform mix as ONE cooperative launch (DFFT_FLAG_BATCH_PERSISTENT),
+    against the grouped launches timed the same way just before it.  Run last:
+    the process's first cooperative launch makes the HIP runtime serialise
+    work across streams from then on."""
+    import numpy as np
+    import torch
+
+    import daggerfft as D
+    from daggerfft import _lib as L
+    from daggerfft.batch import Batch, batch_config_sizes
+    cfg = batch_config_sizes(64, 7)
+    descs = [(shape, L.C2C, -1, prec) for shape, prec in cfg]
+    ins = [D.rand(shape, np.complex128 if prec else np.complex64, seed=1000 + t) for t, (shape, prec) in enumerate(cfg)]
+    outs = [x.similar() for x in ins]
+    res = {}
+    for name, flags in (("grouped", L.FLAG_BATCH_GROUPED), ("persistent", L.FLAG_BATCH_PERSISTENT)):
+        b = Batch(descs, flags=flags)
+        ms = _time_events(lambda: b.execute(ins, outs), 10)
+        res[name] = {"ms": round(ms, 4), "GBps": round(b.algorithmic_bytes() / ms / 1e6, 1), "launches": b.launches}
+    del ins, outs
+    torch.cuda.empty_cache()
+    return res
+
+
 def serving_any_size(count):
     """The same stream with the edges rounded to integers instead of snapped
     to 7-smooth sizes: requests whose edge (or, for R2C, half-edge) has a
@@ -498,7 +526,9 @@ def main():
         print(json.dumps({"serving": serving(args.serve_count)}), flush=True)
         return
     if args.configs_only:
-        print(json.dumps({"configs": configs()}), flush=True)
+        c = configs()
+        c["batch64_mixed_c2c"]["single_launch"] = persistent_batch()
+        print(json.dumps({"configs": c}), flush=True)
         return
     n = args.n
     shape = (n, n, n)
@@ -670,6 +700,11 @@ def main():
             out["serving"] = serving(args.serve_count)
         except Exception as e:  # reported, never fatal
             out["serving"] = {"error": repr(e)}
+    if not args.no_configs and isinstance(out.get("configs"), dict) and "batch64_mixed_c2c" in out["configs"]:
+        try:
+            out["configs"]["batch64_mixed_c2c"]["single_launch"] = persistent_batch()
+        except Exception as e:  # reported, never fatal
+            out["configs"]["batch64_mixed_c2c"]["single_launch"] = {"error": repr(e)}
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(n)

@@ -724,15 +724,18 @@ static std::string batch_key(int count, const dfft_batch_desc *descs) {
 }
 
 // Launch-schedule values of a batch's cache entry (Tuning::variant)
-constexpr int BATCH_PICK_GROUPED = 1, BATCH_PICK_LANES = 2, BATCH_PICK_PERSISTENT = 3;
+constexpr int BATCH_PICK_GROUPED = 1, BATCH_PICK_LANES = 2;
 
 extern "C" int dfft_batch_create(dfft_batch **batch, int count, const dfft_batch_desc *descs, uint32_t flags);
 
-// DFFT_FLAG_MEASURE: times the grouped schedule (when every pass fits it), its
-// single cooperative launch (when every level fits 256-thread tiles) and the
-// lanes schedule (each transform's own plan, its passes autotuned) on scratch
-// arrays of the batch's sizes, best of three, and returns the fastest
-// (BATCH_PICK_*), or a negative DFFT_ERR_* code.
+// DFFT_FLAG_MEASURE: times the grouped schedule (when every pass fits it) and
+// the lanes schedule (each transform's own plan, its passes autotuned) on
+// scratch arrays of the batch's sizes, best of three, and returns the faster
+// (BATCH_PICK_*), or a negative DFFT_ERR_* code.  The single cooperative
+// launch is not a candidate: after a process's first cooperative launch the
+// HIP runtime runs work on different streams one after another (the lanes
+// schedule of the 64-transform mix 2.17 -> 4.03 ms for the rest of the
+// process), so timing it would slow whatever the caller runs next.
 static int measure_batch(int count, const dfft_batch_desc *descs, uint32_t flags) {
     const uint32_t base = flags & ~(uint32_t)(DFFT_FLAG_GRAPH | DFFT_FLAG_BATCH_GROUPED | DFFT_FLAG_BATCH_LANES |
                                               DFFT_FLAG_BATCH_PERSISTENT);
@@ -756,15 +759,13 @@ static int measure_batch(int count, const dfft_batch_desc *descs, uint32_t flags
         else if (hipError_t e3 = hipEventCreate(&e1)) rc = hip_fail(e3, "hipEventCreate");
     }
     if (rc == DFFT_OK && hipDeviceSynchronize() != hipSuccess) rc = hip_fail(hipGetLastError(), "fill");
-    // candidates: grouped, persistent, lanes (BATCH_PICK_* of kind[k])
-    const uint32_t kflag[3] = {DFFT_FLAG_BATCH_GROUPED, DFFT_FLAG_BATCH_PERSISTENT, DFFT_FLAG_BATCH_LANES};
-    const int kind[3] = {BATCH_PICK_GROUPED, BATCH_PICK_PERSISTENT, BATCH_PICK_LANES};
-    float best[3] = {-1.0f, -1.0f, -1.0f};
-    for (int k = 0; k < 3 && rc == DFFT_OK; ++k) {
+    float best[2] = {-1.0f, -1.0f};
+    for (int k = 0; k < 2 && rc == DFFT_OK; ++k) {
         dfft_batch *b = nullptr;
-        const int crc = dfft_batch_create(&b, count, descs, base | kflag[k]);
+        const uint32_t sf = base | (k == 0 ? DFFT_FLAG_BATCH_GROUPED : DFFT_FLAG_BATCH_LANES);
+        const int crc = dfft_batch_create(&b, count, descs, sf);
         if (crc != DFFT_OK) {
-            if (k == 2) rc = crc;  // lanes always builds; a grouped or persistent batch may not
+            if (k == 1) rc = crc;  // lanes always builds; a grouped batch may not
             continue;
         }
         bool ok = dfft_batch_execute(b, in.data(), out.data(), st) == DFFT_OK;  // warm-up
@@ -789,12 +790,8 @@ static int measure_batch(int count, const dfft_batch_desc *descs, uint32_t flags
     }
     if (rc != DFFT_OK) return -rc;
     if (getenv("DFFT_TUNE_VERBOSE"))
-        fprintf(stderr, "tune batch of %d: grouped %.4f ms, persistent %.4f ms, lanes %.4f ms\n", count, best[0],
-                best[1], best[2]);
-    int pick = 2;  // lanes, which always built
-    for (int k = 0; k < 2; ++k)
-        if (best[k] >= 0 && best[k] <= best[pick]) pick = k;
-    return kind[pick];
+        fprintf(stderr, "tune batch of %d: grouped %.4f ms, lanes %.4f ms\n", count, best[0], best[1]);
+    return (best[0] >= 0 && best[0] <= best[1]) ? BATCH_PICK_GROUPED : BATCH_PICK_LANES;
 }
 
 extern "C" {
@@ -828,7 +825,6 @@ int dfft_batch_create(dfft_batch **batch, int count, const dfft_batch_desc *desc
         return DFFT_ERR_ARG;
     }
     b->persistent = (flags & DFFT_FLAG_BATCH_PERSISTENT) != 0;
-    bool persist_cached = false;  // the single launch came from the cache, not a flag
     bool grouped = (flags & (DFFT_FLAG_BATCH_GROUPED | DFFT_FLAG_BATCH_PERSISTENT)) != 0;
     if (!grouped && !(flags & DFFT_FLAG_BATCH_LANES)) {
         // default: the schedule cache's pick for this mix (DFFT_FLAG_MEASURE
@@ -849,10 +845,7 @@ int dfft_batch_create(dfft_batch **batch, int count, const dfft_batch_desc *desc
             have = true;
         }
         if (have) {
-            grouped = pick.variant == BATCH_PICK_GROUPED || pick.variant == BATCH_PICK_PERSISTENT;
-            // a cached single launch only where no graph was asked for
-            b->persistent = pick.variant == BATCH_PICK_PERSISTENT && !(flags & DFFT_FLAG_GRAPH);
-            persist_cached = b->persistent;
+            grouped = pick.variant == BATCH_PICK_GROUPED;
         } else {
             grouped = true;
             for (int t = 0; t < count && grouped; ++t) {
@@ -969,14 +962,7 @@ int dfft_batch_create(dfft_batch **batch, int count, const dfft_batch_desc *desc
         }
     }
     rc = group_create(&b->g, levels);
-    if (rc == DFFT_OK && b->persistent) {
-        rc = group_persistent_prepare(b->g);
-        // a cached pick the device cannot launch falls back to the grouped launches
-        if (rc != DFFT_OK && persist_cached) {
-            b->persistent = false;
-            rc = DFFT_OK;
-        }
-    }
+    if (rc == DFFT_OK && b->persistent) rc = group_persistent_prepare(b->g);
     if (rc == DFFT_OK && (flags & DFFT_FLAG_GRAPH) && !(b->graph = group_graph_new())) rc = DFFT_ERR_OOM;
     if (rc != DFFT_OK) {
         batch_free(b);

@@ -299,13 +299,17 @@ int dfft_plan_export_wisdom(const dfft_plan *plan, const char *path);
  *                            4096-point contiguous axis);
  *                            DFFT_ERR_UNSUPPORTED on a device without
  *                            cooperative launches.  With DFFT_FLAG_BATCH_LANES
- *                            or DFFT_FLAG_GRAPH: DFFT_ERR_ARG.
+ *                            or DFFT_FLAG_GRAPH: DFFT_ERR_ARG.  Opt-in only:
+ *                            measured 25% slower than the grouped launches,
+ *                            and after a process's first cooperative launch
+ *                            the HIP runtime runs work on different streams
+ *                            one after another for the rest of the process
+ *                            (the lanes schedule, the serving loop);
  * No schedule flag: the schedule cache's pick for this mix of descriptors;
- * with DFFT_FLAG_MEASURE and no pick cached, the three schedules are timed on
- * scratch arrays of the batch's sizes (the lanes' plans with their passes
- * autotuned) and the fastest is cached (and saved with the cache when a save
- * path is set).  A cached single-launch pick falls back to the grouped
- * launches with DFFT_FLAG_GRAPH or on a device that cannot launch it.  With no pick: grouped when every axis length of every
+ * with DFFT_FLAG_MEASURE and no pick cached, the grouped and lanes schedules
+ * are timed on scratch arrays of the batch's sizes (the lanes' plans with
+ * their passes autotuned) and the faster is cached (and saved with the cache
+ * when a save path is set).  With no pick: grouped when every axis length of every
  * transform is one of the compile-time grouped lengths and every transform
  * is C2C, else lanes.  DFFT_FLAG_BATCH_GROUPED with DFFT_FLAG_BATCH_LANES:
  * DFFT_ERR_ARG.

@@ -11,7 +11,7 @@ was found, so the kernel was removed and these tests stay skipped.  The
 batch now runs as grouped launches (one per axis-pass level, no waits
 between workgroups), tested by the test_grouped_* functions below, or as
 those launches' tiles in one cooperative launch with grid barriers between
-levels (test_gpu_persistent_batch.py)."""
+levels (test_zy_persistent_batch.py)."""
 import numpy as np
 import pytest
 
@@ -307,11 +307,10 @@ def test_batch_default_schedule(gpu):
 
 
 def test_measured_batch_schedule_is_cached(gpu, tmp_path):
-    """DFFT_FLAG_MEASURE on a batch with no schedule flag times the grouped,
-    persistent (one cooperative launch) and lanes schedules and caches the
-    fastest under the mix's key: a second batch of the same mix without the
-    flag takes the cached schedule, the saved cache holds the entry, and the
-    results equal the oracle's."""
+    """DFFT_FLAG_MEASURE on a batch with no schedule flag times the grouped
+    and lanes schedules and caches the faster under the mix's key: a second
+    batch of the same mix without the flag takes the cached schedule, the
+    saved cache holds the entry, and the results equal the oracle's."""
     import daggerfft as D
     from daggerfft import _lib as L
     from daggerfft.batch import Batch
@@ -325,7 +324,7 @@ def test_measured_batch_schedule_is_cached(gpu, tmp_path):
         n0 = D.tune_entries()
         b = Batch(descs, flags=L.FLAG_MEASURE)
         picked = b.launches
-        assert picked in (1, 3, 15), picked  # persistent: 1; grouped: 3; lanes: 3 per transform
+        assert picked in (3, 15), picked  # grouped: 3 launches; lanes: 3 per transform
         assert D.tune_entries() > n0
         again = Batch(descs)
         assert again.launches == picked

@@ -3,7 +3,12 @@ grouped schedule's tiles, walked by as many workgroups as the device holds at
 once, with a grid barrier between axis-pass levels.  It runs the same tile
 code on the same schedules as the three grouped launches, so its results
 equal theirs bit for bit; the oracle checks a subset (ComplexF64 rel-L2 <=
-1e-12, ComplexF32 vs the FP64 oracle <= 1e-5)."""
+1e-12, ComplexF32 vs the FP64 oracle <= 1e-5).
+
+The file is named to be collected after the other GPU tests (and before the
+no-allocation entry point's): after a process's first cooperative launch
+the HIP runtime runs work on different streams one after another, which
+would slow the multi-stream tests that follow in the same process."""
 import numpy as np
 import pytest
 
