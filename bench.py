@@ -297,7 +297,7 @@ def configs():
         if prec == L.F32:  # FP32 result against the FP64 transform of the same (widened) input
             x64 = D.DeviceArray(shape, np.complex128, tensor=x.tensor.to(torch.complex128))
             worst32 = max(worst32, rel(y.tensor.to(torch.complex128), D.fft(x64).tensor))
-    out["batch64_mixed_c2c"] = {"transforms": 64, "default_schedule": "grouped" if Batch(descs).launches == 3 else "lanes",
+    out["batch64_mixed_c2c"] = {"transforms": 64, "default_schedule": "lanes" if Batch(descs).launches == 3 * len(descs) else "grouped",
                                 **res, "sequential_plans_ms": round(seq, 4),
                                 "f32_rel_l2_vs_f64_max": worst32, "algorithmic_bytes": b.algorithmic_bytes()}
     del ins, outs

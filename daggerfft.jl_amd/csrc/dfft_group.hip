@@ -107,6 +107,33 @@ __device__ __forceinline__ void gfix_tile(int id, const AxisPass &a, const void 
     }
 }
 
+// The same dispatch restricted to one precision (PM 1: ComplexF32, 2:
+// ComplexF64): a kernel holding one precision's schedules only has that
+// precision's register count (ComplexF32 89 VGPRs, 20 waves per CU, against
+// 156 and 12 for both).
+template <int PM>
+__device__ __forceinline__ void gfix_tile_prec(int id, const AxisPass &a, const void *in, void *out, int64_t tile,
+                                               int tid, void *smem) {
+    switch (id) {
+#define DFFT_GFIX_CASE_P(i, L)                                                                    \
+    case 4 * i + 0:                                                                               \
+        if constexpr (PM == 1) gfix_run<float, L, false>(a, in, out, tile, tid, smem);            \
+        break;                                                                                    \
+    case 4 * i + 1:                                                                               \
+        if constexpr (PM == 1) gfix_run<float, L, true>(a, in, out, tile, tid, smem);             \
+        break;                                                                                    \
+    case 4 * i + 2:                                                                               \
+        if constexpr (PM == 2) gfix_run<double, L, false>(a, in, out, tile, tid, smem);           \
+        break;                                                                                    \
+    case 4 * i + 3:                                                                               \
+        if constexpr (PM == 2) gfix_run<double, L, true>(a, in, out, tile, tid, smem);            \
+        break;
+        DFFT_GFIX_LENGTHS(DFFT_GFIX_CASE_P)
+#undef DFFT_GFIX_CASE_P
+    default: break;
+    }
+}
+
 #define DFFT_GFIX_E(T, P, L, C) \
     {L, P, C ? 1 : 0, GFix<T, L, C>::EMAX, GFix<T, L, C>::SPLIT ? 1 : 0, GFix<T, L, C>::TPL, GFix<T, L, C>::RL::S, \
      GFix<T, L, C>::RL::R}
@@ -172,29 +199,47 @@ __global__ __launch_bounds__(THREADS, FIX ? 3 : 1) void k_group(GroupArgs g) {
     group_block<FIX>(g, g.items, g.block_item, (int64_t)blockIdx.x, (int)threadIdx.x, dfft_smem);
 }
 
+// A launch whose items all run compile-time schedules of one precision
+// (PM 1: ComplexF32, 89 VGPRs, five workgroups per CU; 2: ComplexF64, 156
+// VGPRs, three per CU).  (ComplexF32 tiles of 512 threads, for row segments
+// of 32 columns, two workgroups per CU: the 64-transform mix 1.93 -> 2.06 ms.)
+template <int PM>
+__global__ __launch_bounds__(GFIX_THREADS, PM == 1 ? 5 : 3) void k_group_prec(GroupArgs g) {
+    extern __shared__ __align__(16) unsigned char dfft_smem[];
+    const int item = __builtin_amdgcn_readfirstlane(g.block_item[blockIdx.x]);
+    const GroupItem &it = g.items[item];
+    const void *in = uniform_ptr(reinterpret_cast<const unsigned char *>(g.slots[it.in_slot]) + it.in_off);
+    void *out = uniform_ptr(reinterpret_cast<unsigned char *>(g.slots[it.out_slot]) + it.out_off);
+    const int64_t tile = (int64_t)blockIdx.x - it.first_block;
+    const AxisPass a = it.ap;  // a private copy, as in k_group
+    gfix_tile_prec<PM>(__builtin_amdgcn_readfirstlane(it.gfix), a, in, out, tile, (int)threadIdx.x, dfft_smem);
+}
+
 // The whole group as one cooperative launch: every workgroup of the grid is
 // resident at once (hipLaunchCooperativeKernel refuses a grid larger than the
-// device holds), walks the tiles of level l with a stride of the grid size,
-// and meets the others at a grid barrier before level l + 1 reads what level
-// l wrote.  Every workgroup runs every level's loop, so each reaches each of
-// the nlevels - 1 barriers; no workgroup waits on a particular other one.
-// (Tiles dealt from per-level ticket counters instead measured within 2%,
-// and counters would make two launches of one group on different streams
-// race.)
-constexpr int PERSIST_MAX_LEVELS = 4;
+// device holds), walks the tiles of each of the group's launches with a
+// stride of the grid size, and meets the others at a grid barrier wherever
+// the next launch belongs to the next level, before it reads what this level
+// wrote (a level split by precision into two launches needs none between
+// them).  Every workgroup runs every launch's loop, so each reaches each
+// barrier; no workgroup waits on a particular other one.  (Tiles dealt from
+// per-level ticket counters instead measured within 2%, and counters would
+// make two launches of one group on different streams race.)
+constexpr int PERSIST_MAX_LAUNCHES = 8;
 struct PersistArgs {
-    const GroupItem *items;  // every item, level after level
-    const int *block_item;   // every level's block -> item map, concatenated
-    int nlevels;
-    int first_item[PERSIST_MAX_LEVELS];
-    int64_t map_off[PERSIST_MAX_LEVELS];
-    int64_t nblocks[PERSIST_MAX_LEVELS];
+    const GroupItem *items;  // every item, launch after launch
+    const int *block_item;   // every launch's block -> item map, concatenated
+    int nlaunch;
+    int sync_after[PERSIST_MAX_LAUNCHES];  // 1: the next launch is the next level
+    int first_item[PERSIST_MAX_LAUNCHES];
+    int64_t map_off[PERSIST_MAX_LAUNCHES];
+    int64_t nblocks[PERSIST_MAX_LAUNCHES];
     void *slots[GROUP_MAX_SLOTS];
 };
 
 __global__ __launch_bounds__(GFIX_THREADS, 3) void k_group_persist(PersistArgs g) {
     extern __shared__ __align__(16) unsigned char dfft_smem[];
-    for (int l = 0; l < g.nlevels; ++l) {
+    for (int l = 0; l < g.nlaunch; ++l) {
         const GroupItem *items = g.items + g.first_item[l];
         const int *map = g.block_item + g.map_off[l];
         const int nb = (int)g.nblocks[l];  // <= 0x7fffffff - grid (group_persistent_prepare)
@@ -211,7 +256,7 @@ __global__ __launch_bounds__(GFIX_THREADS, 3) void k_group_persist(PersistArgs g
         // the grid barrier orders memory too: after a workgroup barrier its
         // arriving workitem writes the L2 back, and invalidates it once all
         // workgroups have arrived
-        if (l + 1 < g.nlevels) cooperative_groups::this_grid().sync();
+        if (g.sync_after[l]) cooperative_groups::this_grid().sync();
     }
 }
 
@@ -223,6 +268,8 @@ struct GroupLaunch {
     int64_t nblocks = 0;
     int threads = 256;
     bool fixed = false;  // holds compile-time schedules (k_group<GFIX_THREADS, true>)
+    int prec = 0;        // 1 / 2: compile-time schedules of ComplexF32 / ComplexF64 only (k_group_prec)
+    int level = 0;       // the level (dependency step) the launch belongs to
     size_t lds = 0;
     int64_t map_off = 0;  // offset of this launch's block -> item map
 };
@@ -243,7 +290,9 @@ void group_destroy(GroupExec *g) {
     delete g;
 }
 
-static const void *group_kernel(int threads, bool fixed) {
+static const void *group_kernel(int threads, bool fixed, int prec) {
+    if (prec == 1) return (const void *)k_group_prec<1>;
+    if (prec == 2) return (const void *)k_group_prec<2>;
     if (fixed) return (const void *)k_group<GFIX_THREADS, true>;
     switch (threads) {
     case 1024: return (const void *)k_group<1024, false>;
@@ -258,21 +307,19 @@ int group_create(GroupExec **out, const std::vector<std::vector<GroupItem>> &lev
     if (!g) return DFFT_ERR_OOM;
     std::vector<int> map;
     const int lim = axis_lds_limit();
-    for (const auto &lv : levels) {
-        if (lv.empty()) continue;
+    // one launch of the items `lv` (already checked) at level `level`;
+    // prec 1 / 2: all of them run compile-time ComplexF32 / ComplexF64 schedules
+    auto add_launch = [&](const std::vector<GroupItem> &lv, int level, int prec) -> int {
         GroupLaunch L;
         L.first_item = (int)g->items.size();
         L.nitems = (int)lv.size();
         L.map_off = (int64_t)map.size();
+        L.level = level;
+        L.prec = prec;
         int need = 1;
         bool fixed = false;
         for (size_t i = 0; i < lv.size(); ++i) {
             GroupItem it = lv[i];
-            if (it.in_slot < 0 || it.in_slot >= GROUP_MAX_SLOTS || it.out_slot < 0 || it.out_slot >= GROUP_MAX_SLOTS ||
-                it.ntiles < 1 || it.threads < 1 || it.threads > 1024 || it.gfix >= (int)(sizeof(g_gfix) / sizeof(g_gfix[0]))) {
-                group_destroy(g);
-                return DFFT_ERR_ARG;
-            }
             fixed = fixed || it.gfix >= 0;
             it.first_block = L.nblocks;
             for (int64_t b = 0; b < it.ntiles; ++b) map.push_back((int)i);
@@ -282,13 +329,45 @@ int group_create(GroupExec **out, const std::vector<std::vector<GroupItem>> &lev
             g->items.push_back(it);
         }
         L.threads = need <= 256 ? 256 : need <= 512 ? 512 : 1024;
-        // the compile-time schedules are compiled into the 256-thread kernel only
+        // the compile-time schedules are compiled into the 256-thread kernels only
         L.fixed = fixed;
-        if (L.lds > (size_t)lim || L.nblocks > 0x7fffffffLL || (fixed && L.threads != GFIX_THREADS)) {
-            group_destroy(g);
+        if (L.lds > (size_t)lim || L.nblocks > 0x7fffffffLL || (fixed && L.threads != GFIX_THREADS))
             return DFFT_ERR_UNSUPPORTED_SIZE;
-        }
         g->launches.push_back(L);
+        return DFFT_OK;
+    };
+    int level = 0;
+    for (const auto &lv : levels) {
+        if (lv.empty()) continue;
+        bool all_fixed = true;
+        for (const GroupItem &it : lv) {
+            if (it.in_slot < 0 || it.in_slot >= GROUP_MAX_SLOTS || it.out_slot < 0 || it.out_slot >= GROUP_MAX_SLOTS ||
+                it.ntiles < 1 || it.threads < 1 || it.threads > 1024 ||
+                it.gfix >= (int)(sizeof(g_gfix) / sizeof(g_gfix[0]))) {
+                group_destroy(g);
+                return DFFT_ERR_ARG;
+            }
+            all_fixed = all_fixed && it.gfix >= 0 && it.threads <= GFIX_THREADS;
+        }
+        int rc = DFFT_OK;
+        if (all_fixed) {
+            // a level of compile-time schedules only: one launch per precision,
+            // ComplexF64 first, each in the kernel holding that precision's
+            // schedules alone (fewer registers, more waves per CU)
+            for (int prec = 1; prec >= 0 && rc == DFFT_OK; --prec) {
+                std::vector<GroupItem> part;
+                for (const GroupItem &it : lv)
+                    if (it.prec == prec) part.push_back(it);
+                if (!part.empty()) rc = add_launch(part, level, prec ? 2 : 1);
+            }
+        } else {
+            rc = add_launch(lv, level, 0);
+        }
+        if (rc != DFFT_OK) {
+            group_destroy(g);
+            return rc;
+        }
+        ++level;
     }
     if (g->items.empty()) {
         group_destroy(g);
@@ -306,7 +385,7 @@ int group_create(GroupExec **out, const std::vector<std::vector<GroupItem>> &lev
         return DFFT_ERR_HIP;
     }
     for (const auto &L : g->launches)
-        if (L.lds > 65536 && hipFuncSetAttribute(group_kernel(L.threads, L.fixed), hipFuncAttributeMaxDynamicSharedMemorySize,
+        if (L.lds > 65536 && hipFuncSetAttribute(group_kernel(L.threads, L.fixed, L.prec), hipFuncAttributeMaxDynamicSharedMemorySize,
                                                  lim) != hipSuccess) {
             group_destroy(g);
             return DFFT_ERR_HIP;
@@ -322,7 +401,11 @@ int group_launch(const GroupExec *g, void *const *slots, int nslots, hipStream_t
     for (const auto &L : g->launches) {
         a.items = g->d_items + L.first_item;
         a.block_item = g->d_map + L.map_off;
-        if (L.fixed)
+        if (L.prec == 1)
+            hipLaunchKernelGGL(k_group_prec<1>, dim3((unsigned)L.nblocks), dim3(GFIX_THREADS), L.lds, st, a);
+        else if (L.prec == 2)
+            hipLaunchKernelGGL(k_group_prec<2>, dim3((unsigned)L.nblocks), dim3(GFIX_THREADS), L.lds, st, a);
+        else if (L.fixed)
             hipLaunchKernelGGL((k_group<GFIX_THREADS, true>), dim3((unsigned)L.nblocks), dim3(GFIX_THREADS), L.lds, st,
                                a);
         else if (L.threads == 1024)
@@ -337,7 +420,7 @@ int group_launch(const GroupExec *g, void *const *slots, int nslots, hipStream_t
 }
 
 int group_persistent_prepare(GroupExec *g) {
-    if (g->launches.empty() || g->launches.size() > (size_t)PERSIST_MAX_LEVELS) return DFFT_ERR_UNSUPPORTED;
+    if (g->launches.empty() || g->launches.size() > (size_t)PERSIST_MAX_LAUNCHES) return DFFT_ERR_UNSUPPORTED;
     size_t lds = 0;
     int64_t most = 0;
     for (const auto &L : g->launches) {
@@ -382,8 +465,9 @@ int group_launch_persistent(const GroupExec *g, void *const *slots, int nslots, 
     PersistArgs a{};
     a.items = g->d_items;
     a.block_item = g->d_map;
-    a.nlevels = (int)g->launches.size();
-    for (int l = 0; l < a.nlevels; ++l) {
+    a.nlaunch = (int)g->launches.size();
+    for (int l = 0; l < a.nlaunch; ++l) {
+        a.sync_after[l] = l + 1 < a.nlaunch && g->launches[l + 1].level != g->launches[l].level;
         a.first_item[l] = g->launches[l].first_item;
         a.map_off[l] = g->launches[l].map_off;
         a.nblocks[l] = g->launches[l].nblocks;
@@ -437,7 +521,7 @@ int group_launch_graph(const GroupExec *g, GroupGraph *gg, void *const *slots, i
             a.block_item = g->d_map + L.map_off;
             void *kargs[] = {&a};
             hipKernelNodeParams kp{};
-            kp.func = const_cast<void *>(group_kernel(L.threads, L.fixed));
+            kp.func = const_cast<void *>(group_kernel(L.threads, L.fixed, L.prec));
             kp.gridDim = dim3((unsigned)L.nblocks);
             kp.blockDim = dim3(L.threads);
             kp.sharedMemBytes = (unsigned)L.lds;
@@ -920,7 +1004,8 @@ int dfft_batch_create(dfft_batch **batch, int count, const dfft_batch_desc *desc
     // schedules, level by level: the grouped kernel's compile-time schedules
     // where it has one (256-thread tiles); if a pass of the level does not
     // fit 256 threads, the whole level runs the runtime engine in wider
-    // workgroups
+    // workgroups.  (A level whose passes all take compile-time schedules runs
+    // one launch per precision: group_create.)
     std::vector<std::vector<int>> lv_g(3);
     for (int l = 0; l < 3 && rc == DFFT_OK; ++l) {
         std::vector<PassPlan> sched = lv[l];
