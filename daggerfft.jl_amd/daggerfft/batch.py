@@ -1,7 +1,8 @@
 """Batched launch: many independent 3D transforms per call (dfft_batch_* of
-include/daggerfft_hip.h) -- three grouped launches per batch, one per
-axis-pass level, or each transform's own plan on four forked streams (the
-library picks unless a schedule flag says).  No reference counterpart; the
+include/daggerfft_hip.h) -- grouped launches, one per axis-pass level and
+precision, the same tiles in one cooperative launch, or each transform's own
+plan on four forked streams (the library picks unless a schedule flag
+says).  No reference counterpart; the
 reference runs one cuFFT plan per chunk (src/fftgpu.jl:118-124)."""
 from __future__ import annotations
 

@@ -96,7 +96,7 @@ extern "C" {
                                           contiguous axis runs first -- same result, the passes commute */
 #define DFFT_FLAG_PLANE_PIPELINE 0x40u /* Slab: software-pipeline the 2D stage over z-plane chunks
                                           (grouped launches re-reading each chunk from the Infinity Cache) */
-#define DFFT_FLAG_BATCH_GROUPED  0x80u /* batch: one grouped launch per axis-pass level (three launches) */
+#define DFFT_FLAG_BATCH_GROUPED  0x80u /* batch: grouped launches per axis-pass level (three to six) */
 #define DFFT_FLAG_BATCH_LANES    0x100u /* batch: each transform's own plan on four forked streams */
 #define DFFT_FLAG_MEASURE        0x200u /* time the schedules of passes the cache lacks at plan
                                          * creation (FFTW.MEASURE, src/fft.jl:127), and a batch's
@@ -275,15 +275,18 @@ int dfft_plan_export_wisdom(const dfft_plan *plan, const char *path);
  * No reference counterpart: many independent transforms per kernel launch
  * (the reference runs one cuFFT plan per chunk, src/fftgpu.jl:118-124).
  * Each transform is rank 3 over dims (0,1,2); C2C (either sign) or R2C
- * (forward, even n[0]).  Two schedules:
- *   DFFT_FLAG_BATCH_GROUPED  launch l runs axis pass l of every transform, so
- *                            a whole batch takes three launches and no
- *                            workgroup waits for another.  C2C passes of the
- *                            lengths 16, 32, 48, 64, 80, 96, 112, 128, 144,
- *                            160, 192, 224, 240, 256 run compile-time
- *                            schedules in 256-thread tiles; other passes the
- *                            runtime engine.  Results equal each transform
- *                            alone up to rounding;
+ * (forward, even n[0]).  Three schedules:
+ *   DFFT_FLAG_BATCH_GROUPED  level l (axis pass l of every transform) runs as
+ *                            one launch, or as two -- ComplexF64, then
+ *                            ComplexF32, each kernel holding one precision's
+ *                            schedules -- when all its passes run
+ *                            compile-time schedules; three to six launches
+ *                            per batch, and no workgroup waits for another.
+ *                            C2C passes of the lengths 16, 32, 48, 64, 80,
+ *                            96, 112, 128, 144, 160, 192, 224, 240, 256 run
+ *                            compile-time schedules in 256-thread tiles;
+ *                            other passes the runtime engine.  Results equal
+ *                            each transform alone up to rounding;
  *   DFFT_FLAG_BATCH_LANES    each transform's own plan (the compile-time
  *                            kernels) on one of four streams forked from and
  *                            joined back into the caller's stream; results
