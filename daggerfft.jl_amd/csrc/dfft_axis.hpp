@@ -780,7 +780,14 @@ struct NoRadices {};
 // the tile a single-pass launch's workgroup runs (AxisPass::swz)
 __device__ __forceinline__ int64_t pass_tile(const AxisPass &a) {
     const uint64_t b = blockIdx.x;
-    return a.swz > 1 ? (int64_t)((b * (uint64_t)a.swz) % (uint64_t)gridDim.x) : (int64_t)b;
+    if (a.swz > 1) return (int64_t)((b * (uint64_t)a.swz) % (uint64_t)gridDim.x);
+    if (a.swz == SWZ_XCD) {
+        // workgroup b runs on XCD b mod 8 as that XCD's (b / 8)-th; XCD k
+        // takes tiles [k q + min(k, r), ...) with G = 8 q + r: a bijection on [0, G)
+        const uint32_t G = gridDim.x, q = G >> 3, r = G & 7, k = (uint32_t)b & 7, j = (uint32_t)b >> 3;
+        return (int64_t)k * q + (k < r ? k : r) + j;
+    }
+    return (int64_t)b;
 }
 
 // Thread -> (line slot ls, thread-in-line t) and the tile's global bases.

@@ -215,7 +215,8 @@ static std::vector<Tuning> candidates(const PassPlan &pp, int level, int max_thr
         const int var = a.blue == 2 ? 2 : a.variant ? 3 : 0;
         const std::vector<int> tcs = a.contig ? std::vector<int>{2, 4, 6, 8, 12, 16, 24, 32}
                                               : std::vector<int>{8, 16, 32, 64};
-        const std::vector<int> swizzles = a.contig ? std::vector<int>{1} : std::vector<int>{1, 97, 1021};
+        const std::vector<int> swizzles =
+            a.contig ? std::vector<int>{1, SWZ_XCD} : std::vector<int>{1, 97, 1021, SWZ_XCD};
         for (int tc : tcs) {
             if (tc * a.TPL > max_threads) continue;
             for (int sw : swizzles) {
@@ -232,8 +233,9 @@ static std::vector<Tuning> candidates(const PassPlan &pp, int level, int max_thr
     std::vector<int> pads = a.contig ? (level >= 2 ? std::vector<int>{3, 4, 5} : std::vector<int>{3, 4})
                                      : std::vector<int>{0};
     // strided passes also try spreading the tiles that run at one time over
-    // the array (AxisPass::swz; odd multipliers, coprime to power-of-two tile counts)
-    std::vector<int> swizzles = a.contig ? std::vector<int>{1} : std::vector<int>{1, 97, 1021};
+    // the array (AxisPass::swz; odd multipliers, coprime to power-of-two tile
+    // counts); both layouts try contiguous tile ranges per XCD
+    std::vector<int> swizzles = a.contig ? std::vector<int>{1, SWZ_XCD} : std::vector<int>{1, 97, 1021, SWZ_XCD};
     // compile-time schedules of this length: every radix list the catalog
     // holds, times the tile shape
     for (int var : {1, 2, 3}) {
@@ -487,7 +489,7 @@ static int load_file(const char *path, bool quiet) {
         if (!(ss >> key >> rad >> t.tpl >> t.tc >> t.pad_shift)) return DFFT_ERR_ARG;
         if (!(ss >> t.variant)) t.variant = 0;
         if (!(ss >> t.swz)) t.swz = 1;
-        if (t.variant < 0 || t.variant > 3 || t.swz < 1) return DFFT_ERR_ARG;
+        if (t.variant < 0 || t.variant > 3 || (t.swz < 1 && t.swz != SWZ_XCD)) return DFFT_ERR_ARG;
         if (rad != "-") {
             std::stringstream rs(rad);
             std::string tok;

@@ -7,6 +7,8 @@
 namespace dfft {
 
 constexpr int MAX_STAGES = 14;
+// AxisPass::swz: contiguous tile ranges per XCD (gfx950: 8 XCDs, workgroups dealt round robin)
+constexpr int SWZ_XCD = -8;
 
 // Operation fused into an axis pass.
 enum PassOp : int {
@@ -64,7 +66,10 @@ struct AxisPass {
     int swz;                  // workgroup b runs tile (b * swz) mod (number of tiles); 1 = in order.
                               // Coprime to the tile count, so every tile runs once; spreads the
                               // tiles running at one time over the array (HBM channel balance
-                              // for lines with a large power-of-two stride)
+                              // for lines with a large power-of-two stride).  SWZ_XCD: each of
+                              // the 8 XCDs (which take workgroups round robin) runs one
+                              // contiguous range of tiles, so tiles that share cache lines
+                              // (rows not aligned to 128 B) share that XCD's L2
     int64_t nlines;           // contig: number of lines; strided: number of columns
     int64_t nouter;           // strided: number of outer slices (1 for contig)
     int64_t in_sm, in_so;     // input: stride along the line, stride between lines/outer
