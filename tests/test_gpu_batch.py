@@ -346,3 +346,25 @@ def test_measured_batch_schedule_is_cached(gpu, tmp_path):
     finally:
         D.tune_clear()
         D.tune_load(shipped)
+
+
+@pytest.mark.parametrize("descs", [
+    [((64, 48, 32), 0, -1, 1)],                                      # one transform
+    [((1, 16, 1), 0, -1, 1), ((2, 1, 3), 0, +1, 0), ((32, 1, 48), 0, -1, 0)],  # axes of length 1
+    [((96, 64, 32), 0, -1, 0), ((96, 64, 32), 0, -1, 1), ((30, 64, 32), 0, -1, 1)],  # levels split and not
+])
+def test_grouped_batch_edge_mixes(gpu, descs):
+    """Grouped launches (one per level, or per level and precision where the
+    level's passes all run compile-time schedules) on a single transform,
+    axes of length 1, and a mix whose first level holds a runtime-engine pass
+    (30 points) while the others split by precision: every result within
+    rounding of the same transform planned alone, and the oracle."""
+    import daggerfft as D
+    b, ins, outs = _run_grouped(descs, [90 + t for t in range(len(descs))])
+    for t, ((shape, kind, sign, prec), X, Y) in enumerate(zip(descs, ins, outs)):
+        alone = D.fft(X) if sign < 0 else D.ifft(X)
+        assert O.rel_l2(Y.to_host(), alone.to_host()) <= (1e-14 if prec == 1 else 1e-6), t
+        dt = np.complex128 if prec == 1 else np.complex64
+        a = O.fill(90 + t, shape, dt).astype(np.complex128)
+        e = O.rel_l2(Y.to_host().astype(np.complex128), O.fftn(a, inverse=sign > 0))
+        assert e <= (1e-12 if prec == 1 else 1e-5), (t, shape, e)

@@ -122,3 +122,19 @@ def test_persistent_batch_flag_errors(gpu):
         with pytest.raises(L.DFFTError) as ei:
             Batch(_small_mix(), flags=L.FLAG_BATCH_PERSISTENT | extra)
         assert ei.value.status == L.DFFT_ERR_ARG
+
+
+def test_persistent_batch_edge_mixes(gpu):
+    """The single cooperative launch on a one-transform batch and on axes of
+    length 1: equal to the grouped launches bit for bit."""
+    from daggerfft import _lib as L
+    from daggerfft.batch import Batch
+    for descs in ([((64, 48, 32), L.C2C, -1, 1)],
+                  [((1, 16, 1), L.C2C, -1, 1), ((2, 1, 3), L.C2C, +1, 0), ((32, 1, 48), L.C2C, -1, 0)]):
+        print("persistent edge mix", [d[0] for d in descs], flush=True)
+        p = Batch(descs, flags=L.FLAG_BATCH_PERSISTENT)
+        g = Batch(descs, flags=L.FLAG_BATCH_GROUPED)
+        assert p.launches == 1
+        ins = _inputs(descs, [90 + t for t in range(len(descs))])
+        for a, b in zip(_run(p, descs, ins), _run(g, descs, ins)):
+            assert np.array_equal(a, b)
