@@ -282,7 +282,7 @@ def configs():
     res = {}
     # (the single cooperative launch is timed at the very end of the run,
     # persistent_batch(): after a process's first cooperative launch the HIP
-    # runtime serialises work across streams, which would slow the lanes and
+    # runtime runs multi-stream work slower, which would slow the lanes and
     # the serving loop measured after it)
     for name, flags in (("grouped", L.FLAG_BATCH_GROUPED), ("grouped_graph", L.FLAG_BATCH_GROUPED | L.FLAG_GRAPH),
                         ("lanes", L.FLAG_BATCH_LANES), ("lanes_graph", L.FLAG_BATCH_LANES | L.FLAG_GRAPH)):
@@ -444,8 +444,8 @@ def _smooth(m):
 def persistent_batch():
     """The 64-transform mix as ONE cooperative launch (DFFT_FLAG_BATCH_PERSISTENT),
     against the grouped launches timed the same way just before it.  Run last:
-    the process's first cooperative launch makes the HIP runtime serialise
-    work across streams from then on."""
+    after the process's first cooperative launch the HIP runtime runs work
+    spread over several streams slower from then on."""
     import numpy as np
     import torch
 

@@ -816,10 +816,10 @@ extern "C" int dfft_batch_create(dfft_batch **batch, int count, const dfft_batch
 // the lanes schedule (each transform's own plan, its passes autotuned) on
 // scratch arrays of the batch's sizes, best of three, and returns the faster
 // (BATCH_PICK_*), or a negative DFFT_ERR_* code.  The single cooperative
-// launch is not a candidate: after a process's first cooperative launch the
-// HIP runtime runs work on different streams one after another (the lanes
-// schedule of the 64-transform mix 2.17 -> 4.03 ms for the rest of the
-// process), so timing it would slow whatever the caller runs next.
+// launch is not a candidate: after a process's first cooperative launch,
+// work spread over several streams runs slower for the rest of the process
+// (the lanes schedule of the 64-transform mix 2.17 -> 4.03 ms), so timing
+// it would slow whatever the caller runs next.
 static int measure_batch(int count, const dfft_batch_desc *descs, uint32_t flags) {
     const uint32_t base = flags & ~(uint32_t)(DFFT_FLAG_GRAPH | DFFT_FLAG_BATCH_GROUPED | DFFT_FLAG_BATCH_LANES |
                                               DFFT_FLAG_BATCH_PERSISTENT);

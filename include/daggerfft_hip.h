@@ -305,9 +305,10 @@ int dfft_plan_export_wisdom(const dfft_plan *plan, const char *path);
  *                            or DFFT_FLAG_GRAPH: DFFT_ERR_ARG.  Opt-in only:
  *                            measured 25% slower than the grouped launches,
  *                            and after a process's first cooperative launch
- *                            the HIP runtime runs work on different streams
- *                            one after another for the rest of the process
- *                            (the lanes schedule, the serving loop);
+ *                            work spread over several streams runs slower
+ *                            for the rest of the process (measured: the
+ *                            lanes schedule 2.17 -> 4.03 ms; it concerns the
+ *                            serving loop too);
  * No schedule flag: the schedule cache's pick for this mix of descriptors;
  * with DFFT_FLAG_MEASURE and no pick cached, the grouped and lanes schedules
  * are timed on scratch arrays of the batch's sizes (the lanes' plans with

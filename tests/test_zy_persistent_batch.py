@@ -6,9 +6,9 @@ equal theirs bit for bit; the oracle checks a subset (ComplexF64 rel-L2 <=
 1e-12, ComplexF32 vs the FP64 oracle <= 1e-5).
 
 The file is named to be collected after the other GPU tests (and before the
-no-allocation entry point's): after a process's first cooperative launch
-the HIP runtime runs work on different streams one after another, which
-would slow the multi-stream tests that follow in the same process."""
+no-allocation entry point's): after a process's first cooperative launch,
+work spread over several streams runs slower for the rest of the process,
+which would slow the multi-stream tests that follow in it."""
 import numpy as np
 import pytest
 
