@@ -1044,13 +1044,16 @@ struct RaderCtx : TileCtx<T, OP, CONTIG> {
     }
 };
 
-// A contiguous line's permuted reads and writes, staged through the line's
-// LDS slot (complex values; the tile's LDS is sized for them): the line moves
-// between global memory and LDS in natural order, one coalesced burst, and
-// the permutations x_{g^q} / X_{g^-p} become LDS addressing.
+// A contiguous line's permuted reads and writes, staged through LDS (complex
+// values; the tile's LDS is sized for them): the lines move between global
+// memory and LDS in natural order, coalesced, and the permutations x_{g^q} /
+// X_{g^-p} become LDS addressing.  The staging is columnar (Base's layout is
+// the strided one, element p of line ls at p * P + ls): lanes run across the
+// tile's lines, so each gather, scatter and FFT exchange touches consecutive
+// LDS words (a per-line slot took bank conflicts on the permuted accesses).
 template <class T, int OP, int PHASE>
-struct RaderStagedCtx : RaderCtx<T, OP, true, PHASE> {
-    typedef RaderCtx<T, OP, true, PHASE> Base;
+struct RaderStagedCtx : RaderCtx<T, OP, false, PHASE> {
+    typedef RaderCtx<T, OP, false, PHASE> Base;
     __device__ __forceinline__ RaderStagedCtx(const typename Base::Op &c, cx<T> x0_) : Base(c, x0_) {}
     __device__ __forceinline__ cx<T> load0(int p) const { return this->lds[this->li((int)this->perm[p].x)]; }
     __device__ __forceinline__ void store_last(int p, cx<T> v) const {
@@ -1064,42 +1067,59 @@ __device__ __forceinline__ void rader_fixed_tile(const AxisPass &a, const void *
     static_assert(OP == OP_C2C, "Rader passes are C2C");
     constexpr int EMAX = 16, TPLC = fixed_tpl<RL, EMAX>(), M = RL::L, N = M + 1;
     using RR = Reversed<RL>;
-    int ls, t;
-    bool valid;
-    int64_t ib, ob;
-    tile_map<CONTIG>(a, tile, tid, TPLC, ls, t, valid, ib, ob);
-    TileCtx<T, OP, CONTIG> c{a, in, out, lds, ls, t, valid, ib, ob};
     cx<T> v[EMAX];
     if constexpr (CONTIG) {
-        // the whole line into its LDS slot in natural order
-        constexpr int NPT = (N + TPLC - 1) / TPLC;
-#pragma unroll
-        for (int k = 0; k < NPT; ++k) {
-            const int i = t + k * TPLC;
-            if (i < N) lds[c.li(i)] = c.load0(i);
-        }
+        // Columnar tile of TC contiguous lines: thread tid works on line
+        // ls = tid % TC as its thread t = tid / TC, and element i of line ls
+        // sits in LDS at i * P + ls (P = a.lds_pitch, odd and >= TC; the
+        // planner sizes the LDS for N * P complex values).
+        const int TC = a.TC, P = a.lds_pitch, NT = TPLC * TC;
+        const bool active = tid < NT;
+        const int ls = active ? tid % TC : 0;
+        const int t = active ? tid / TC : (1 << 28);  // inactive: beyond every butterfly
+        const int64_t line0 = tile * (int64_t)TC;
+        const bool valid = active && line0 + ls < a.nlines;
+        // the op's load and store maps of line `line` of the tile
+        auto line_ctx = [&](int line) {
+            const int64_t g = line0 + line;
+            return TileCtx<T, OP, true>{a, in, out, lds, line, 0, g < a.nlines, g * a.in_so, g * a.out_so};
+        };
+        // the tile's lines into LDS in natural order, flattened over the lines
+        if (active)
+            for (int f = tid; f < TC * N; f += NT) {
+                const int line = f / N, i = f - line * N;
+                lds[i * P + line] = line_ctx(line).load0(i);
+            }
         __syncthreads();
-        const cx<T> x0 = t < (1 << 28) ? lds[c.li(0)] : cx<T>{0, 0};
+        const TileCtx<T, OP, false> c{a, in, out, lds, ls, t, valid, 0, 0};  // LDS: p * P + ls
+        const cx<T> x0 = active ? lds[ls] : cx<T>{0, 0};
         const RaderStagedCtx<T, OP, 0> c0(c, x0);
         const RaderStagedCtx<T, OP, 1> c1(c, x0);
         st_load_global<RL::R[0], EMAX, M, TPLC>(c0, v);
-        __syncthreads();  // the first exchange reuses the slot
+        __syncthreads();  // the first exchange reuses the staging area
         st_compute<RL::R[0], EMAX, M, TPLC, 1>(c0, v, 0);
         fixed_stages_from<RL, EMAX, TPLC, SPLIT, 0>(c0, v);
         const cx<T> a0 = v[0];
         blue_turn<RL::R[RL::S - 1], EMAX, M, TPLC>(c0, v);
         st_compute<RR::R[0], EMAX, M, TPLC, 1>(c1, v, 0);
         fixed_stages_from<RR, EMAX, TPLC, SPLIT, 0>(c1, v);
-        // the last exchange ended with a barrier: X into the slot at g^-p, then out in natural order
+        // the last exchange ended with a barrier (or, with a one-stage FFT, the
+        // barrier after the gather): X into LDS at g^-p, then out in natural order
         st_store_global<RR::R[RR::S - 1], EMAX, M, TPLC>(c1, v);
-        if (t == 0) lds[c.li(0)] = x0 + a0;
+        if (t == 0) lds[ls] = x0 + a0;
         __syncthreads();
-#pragma unroll
-        for (int k = 0; k < NPT; ++k) {
-            const int i = t + k * TPLC;
-            if (i < N) c.store_last(i, lds[c.li(i)]);
-        }
+        if (active)
+            for (int f = tid; f < TC * N; f += NT) {
+                const int line = f / N, i = f - line * N;
+                const auto lc = line_ctx(line);
+                if (lc.valid) lc.store_last(i, lds[i * P + line]);
+            }
     } else {
+        int ls, t;
+        bool valid;
+        int64_t ib, ob;
+        tile_map<CONTIG>(a, tile, tid, TPLC, ls, t, valid, ib, ob);
+        TileCtx<T, OP, CONTIG> c{a, in, out, lds, ls, t, valid, ib, ob};
         const cx<T> x0 = t < (1 << 28) ? c.load0(0) : cx<T>{0, 0};
         const RaderCtx<T, OP, CONTIG, 0> c0(c, x0);
         const RaderCtx<T, OP, CONTIG, 1> c1(c, x0);

@@ -450,9 +450,10 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
     }
     const int emax = a.variant >= 2 ? 16 : EMAX;
     // split exchange: half the LDS; a contiguous Rader line stages whole
-    // complex values in its slot for its permuted gather and scatter
-    // (measured, 257 FP64: 0.331 ms scattered, 0.297 ms staged; staging real
-    // and imaginary parts in two rounds through half the slot, 0.314 ms)
+    // complex values for its permuted gather and scatter (measured, 257 FP64:
+    // 0.331 ms scattered, 0.297 ms staged in a slot per line; staging real and
+    // imaginary parts in two rounds through half the slot, 0.314 ms; the
+    // columnar staging is sized below)
     const int lds_es = (a.variant == 2 && !(a.blue == 2 && a.contig)) ? es / 2 : es;
     int tpl = a.variant ? fixed_tpl
               : (tune && tune->variant == 0 && tune->tpl > 0 && tpl_ok(L, R, emax, tune->tpl))
@@ -463,7 +464,26 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
     a.TPL = tpl;
     const int rows = L + 1;  // C2R stages H+1 bins
     const int lds_limit = max_lds ? (int)std::min<size_t>(max_lds, (size_t)axis_lds_limit()) : axis_lds_limit();
-    if (a.contig) {
+    if (a.contig && a.blue == 2) {
+        // Rader along contiguous lines: a columnar tile of TC lines whose LDS
+        // holds element i of line ls at i * P + ls, P = TC rounded up to odd,
+        // for the n staged complex values of each line (rader_fixed_tile)
+        const int es_c = pp.prec ? 16 : 8;
+        // (default: about 192 threads per tile; tuned, 257 took 12 lines of 16
+        // threads and 251 took 4 of 25)
+        int tc = (tune && tune->tc > 0) ? tune->tc : (conv_tune && conv_tune->tc > 0) ? conv_tune->tc
+                                                                                     : std::max(1, 192 / tpl);
+        auto lds_of = [&](int c) { return (size_t)a.sp.L * (size_t)(c | 1) * es_c; };
+        while (tc > 1 && (int64_t)tc * tpl > max_threads) --tc;
+        while (tc > 1 && lds_of(tc) > (size_t)lds_limit) --tc;
+        if (lds_of(tc) > (size_t)lds_limit || (int64_t)tc * tpl > max_threads) return DFFT_ERR_UNSUPPORTED_SIZE;
+        a.pad_shift = 30;
+        a.TC = tc;
+        a.lds_pitch = tc | 1;
+        pp.threads = tpl * tc;
+        pp.lds = lds_of(tc);
+        pp.nblocks = (a.nlines + tc - 1) / tc;
+    } else if (a.contig) {
         a.pad_shift = (tune && tune->pad_shift > 0) ? tune->pad_shift : (pp.prec ? 3 : 4);
         a.lds_pitch = rows + (rows >> a.pad_shift) + 1;
         // 128-thread workgroups by default (64 for the split schedule): many
