@@ -1047,13 +1047,15 @@ struct RaderCtx : TileCtx<T, OP, CONTIG> {
 // A contiguous line's permuted reads and writes, staged through LDS (complex
 // values; the tile's LDS is sized for them): the lines move between global
 // memory and LDS in natural order, coalesced, and the permutations x_{g^q} /
-// X_{g^-p} become LDS addressing.  The staging is columnar (Base's layout is
-// the strided one, element p of line ls at p * P + ls): lanes run across the
-// tile's lines, so each gather, scatter and FFT exchange touches consecutive
-// LDS words (a per-line slot took bank conflicts on the permuted accesses).
-template <class T, int OP, int PHASE>
-struct RaderStagedCtx : RaderCtx<T, OP, false, PHASE> {
-    typedef RaderCtx<T, OP, false, PHASE> Base;
+// X_{g^-p} become LDS addressing.  CL false: columnar staging (the strided
+// layout, element p of line ls at p * P + ls): lanes run across the tile's
+// lines, so each gather, scatter and FFT exchange touches consecutive LDS
+// words.  CL true: a padded slot per line (lines of more than 64 threads,
+// whose tiles hold too few lines for the columnar layout; the permuted
+// accesses take bank conflicts there).
+template <class T, int OP, bool CL, int PHASE>
+struct RaderStagedCtx : RaderCtx<T, OP, CL, PHASE> {
+    typedef RaderCtx<T, OP, CL, PHASE> Base;
     __device__ __forceinline__ RaderStagedCtx(const typename Base::Op &c, cx<T> x0_) : Base(c, x0_) {}
     __device__ __forceinline__ cx<T> load0(int p) const { return this->lds[this->li((int)this->perm[p].x)]; }
     __device__ __forceinline__ void store_last(int p, cx<T> v) const {
@@ -1068,7 +1070,7 @@ __device__ __forceinline__ void rader_fixed_tile(const AxisPass &a, const void *
     constexpr int EMAX = 16, TPLC = fixed_tpl<RL, EMAX>(), M = RL::L, N = M + 1;
     using RR = Reversed<RL>;
     cx<T> v[EMAX];
-    if constexpr (CONTIG) {
+    if constexpr (CONTIG && TPLC <= 64) {
         // Columnar tile of TC contiguous lines: thread tid works on line
         // ls = tid % TC as its thread t = tid / TC, and element i of line ls
         // sits in LDS at i * P + ls (P = a.lds_pitch, odd and >= TC; the
@@ -1093,8 +1095,8 @@ __device__ __forceinline__ void rader_fixed_tile(const AxisPass &a, const void *
         __syncthreads();
         const TileCtx<T, OP, false> c{a, in, out, lds, ls, t, valid, 0, 0};  // LDS: p * P + ls
         const cx<T> x0 = active ? lds[ls] : cx<T>{0, 0};
-        const RaderStagedCtx<T, OP, 0> c0(c, x0);
-        const RaderStagedCtx<T, OP, 1> c1(c, x0);
+        const RaderStagedCtx<T, OP, false, 0> c0(c, x0);
+        const RaderStagedCtx<T, OP, false, 1> c1(c, x0);
         st_load_global<RL::R[0], EMAX, M, TPLC>(c0, v);
         __syncthreads();  // the first exchange reuses the staging area
         st_compute<RL::R[0], EMAX, M, TPLC, 1>(c0, v, 0);
@@ -1114,6 +1116,40 @@ __device__ __forceinline__ void rader_fixed_tile(const AxisPass &a, const void *
                 const auto lc = line_ctx(line);
                 if (lc.valid) lc.store_last(i, lds[i * P + line]);
             }
+    } else if constexpr (CONTIG) {
+        int ls, t;
+        bool valid;
+        int64_t ib, ob;
+        tile_map<CONTIG>(a, tile, tid, TPLC, ls, t, valid, ib, ob);
+        TileCtx<T, OP, CONTIG> c{a, in, out, lds, ls, t, valid, ib, ob};
+        // the whole line into its LDS slot in natural order
+        constexpr int NPT = (N + TPLC - 1) / TPLC;
+#pragma unroll
+        for (int k = 0; k < NPT; ++k) {
+            const int i = t + k * TPLC;
+            if (i < N) lds[c.li(i)] = c.load0(i);
+        }
+        __syncthreads();
+        const cx<T> x0 = t < (1 << 28) ? lds[c.li(0)] : cx<T>{0, 0};
+        const RaderStagedCtx<T, OP, true, 0> c0(c, x0);
+        const RaderStagedCtx<T, OP, true, 1> c1(c, x0);
+        st_load_global<RL::R[0], EMAX, M, TPLC>(c0, v);
+        __syncthreads();  // the first exchange reuses the slot
+        st_compute<RL::R[0], EMAX, M, TPLC, 1>(c0, v, 0);
+        fixed_stages_from<RL, EMAX, TPLC, SPLIT, 0>(c0, v);
+        const cx<T> a0 = v[0];
+        blue_turn<RL::R[RL::S - 1], EMAX, M, TPLC>(c0, v);
+        st_compute<RR::R[0], EMAX, M, TPLC, 1>(c1, v, 0);
+        fixed_stages_from<RR, EMAX, TPLC, SPLIT, 0>(c1, v);
+        // the last exchange ended with a barrier: X into the slot at g^-p, then out in natural order
+        st_store_global<RR::R[RR::S - 1], EMAX, M, TPLC>(c1, v);
+        if (t == 0) lds[c.li(0)] = x0 + a0;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NPT; ++k) {
+            const int i = t + k * TPLC;
+            if (i < N) c.store_last(i, lds[c.li(i)]);
+        }
     } else {
         int ls, t;
         bool valid;

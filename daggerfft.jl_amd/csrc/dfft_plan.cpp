@@ -464,16 +464,19 @@ int schedule_pass(PassPlan &pp, const Tuning *tune, int max_threads, size_t max_
     a.TPL = tpl;
     const int rows = L + 1;  // C2R stages H+1 bins
     const int lds_limit = max_lds ? (int)std::min<size_t>(max_lds, (size_t)axis_lds_limit()) : axis_lds_limit();
-    if (a.contig && a.blue == 2) {
-        // Rader along contiguous lines: a columnar tile of TC lines whose LDS
-        // holds element i of line ls at i * P + ls, P = TC rounded up to odd,
-        // for the n staged complex values of each line (rader_fixed_tile)
+    if (a.contig && tpl <= 64 && a.blue == 2) {
+        // Rader along contiguous lines of at most 64 threads: a columnar tile
+        // of TC lines whose LDS holds element i of line ls at i * P + ls, P =
+        // TC rounded up to odd, for the n staged complex values of each line
+        // (rader_fixed_tile).  (The same staging for C2C Bluestein tiles, whose
+        // accesses are not permuted, gained 0-2% and was not kept.)
         const int es_c = pp.prec ? 16 : 8;
         // (default: about 192 threads per tile; tuned, 257 took 12 lines of 16
         // threads and 251 took 4 of 25)
         int tc = (tune && tune->tc > 0) ? tune->tc : (conv_tune && conv_tune->tc > 0) ? conv_tune->tc
                                                                                      : std::max(1, 192 / tpl);
-        auto lds_of = [&](int c) { return (size_t)a.sp.L * (size_t)(c | 1) * es_c; };
+        const size_t per_row = std::max((size_t)a.sp.L * es_c, (size_t)a.bsp.L * lds_es);
+        auto lds_of = [&](int c) { return per_row * (size_t)(c | 1); };
         while (tc > 1 && (int64_t)tc * tpl > max_threads) --tc;
         while (tc > 1 && lds_of(tc) > (size_t)lds_limit) --tc;
         if (lds_of(tc) > (size_t)lds_limit || (int64_t)tc * tpl > max_threads) return DFFT_ERR_UNSUPPORTED_SIZE;
