@@ -9,9 +9,12 @@ One "step" is one forward plus one inverse transform of the resident
 counter generator, seed 3).  Both decompositions are measured:
   * Pencil: one launch per axis pass; every pass is bracketed by HIP events
     on the stream it runs on (per-pass GB/s against the 8 TB/s HBM peak);
-  * Slab: the 2D stage software-pipelined over chunks of z-planes in grouped
-    launches, then the third axis; each transform bracketed by HIP events.
-`value` is the faster decomposition (named in config.decomp).
+  * Slab: each transform bracketed by HIP events.  On one GPU a Slab plan
+    runs the same three axis passes as a Pencil plan (its plane-pipelined 2D
+    stage is opt-in, DFFT_FLAG_PLANE_PIPELINE, and measured slower), so the
+    two are two timings of the same work.
+`value` is the faster of the two (named in config.decomp; config.decomp_note
+says they run the same passes).
 
 Prints one JSON line.
 """
@@ -654,6 +657,8 @@ def main():
                     "bytes_per_launch": dom["bytes"], "avg_ms": dom["avg_ms"]}
 
     cfg = {"workload": "3D C2C ComplexF64 %dx%dx%d forward+inverse" % (n, n, n), "decomp": best,
+           "decomp_note": "on one GPU Pencil and Slab run the same three axis passes (the Slab plane "
+                          "pipeline is opt-in); value is the faster of the two timings of that work",
            "dims": [1, 2, 3], "step": "fft then ifft", "time_per_transform_ms": round(ms_step / 2, 4),
            "hbm_frac_per_transform": round(r["bytes"] / t_transform_s / 1e9 / HBM_PEAK_GBPS, 4),
            "algorithmic_bytes_per_transform": r["bytes"], "roundtrip_rel_l2": r["err"],
