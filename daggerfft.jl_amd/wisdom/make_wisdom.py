@@ -57,7 +57,7 @@ def main():
     from daggerfft.batch import Batch, batch_config_sizes
     descs = [(shape, L.C2C, -1, prec) for shape, prec in batch_config_sizes(64, 7)]
     b = Batch(descs, flags=L.FLAG_MEASURE)
-    print("batch of 64: %s schedule  %.1fs  entries %d" % ("grouped" if b.launches == 3 else "lanes",
+    print("batch of 64: %s schedule  %.1fs  entries %d" % ("lanes" if b.launches == 3 * len(descs) else "grouped",
                                                           time.time() - t0, D.tune_entries()), flush=True)
     b.destroy()
     if out is None:
