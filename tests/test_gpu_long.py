@@ -5,7 +5,8 @@ the contiguous axis, a strided axis with outer slices and a strided axis
 without, in both precisions; FP64 4096 through both of its radix lists.  The
 fused C2R (split exchange) of every smooth half-length 257..1024
 (dfft_fixed_real*.hip) likewise, with the R2C of the same lines (runtime
-engine).
+engine).  Full-size long-line shapes whose tuned schedules the shipped cache
+holds, every element against the oracle.
 ComplexF64 rel-L2 <= 1e-12; ComplexF32 against the FP64 oracle <= 2e-6."""
 import math
 import os
@@ -122,6 +123,47 @@ def test_f64_4096_both_radix_lists(gpu, tmp_path):
             assert e <= 1e-12 and e2 <= 1e-12, (shape, e, e2)
     finally:
         D.tune_clear()
+
+
+SHIPPED_LONG = [
+    # shape, axis whose line is long; the shipped schedule cache holds tuned
+    # entries for these sizes (a strided 4096-point pass runs one contiguous
+    # range of tiles per XCD)
+    ((256, 4096, 128), 1),
+    ((128, 128, 4096), 2),
+    ((16384, 64, 64), 0),
+]
+
+
+@pytest.mark.parametrize("case", range(len(SHIPPED_LONG)))
+def test_long_lines_shipped_schedules_full_array(gpu, case):
+    """Full-size long-line transforms with the shipped schedule cache
+    (64M-134M points): every output element of the forward transform against
+    the oracle, and the inverse back to the input."""
+    import daggerfft as D
+    from daggerfft import _lib as L
+    shape, ax = SHIPPED_LONG[case]
+    print("long, shipped schedules", shape, flush=True)
+    D.tune_clear()
+    D.tune_load(os.path.join(os.path.dirname(D.__file__), "..", "wisdom", "gfx950.txt"))
+    a = O.fill(7 + case, shape, np.complex128)
+    p = D.Plan(shape, L.C2C, -1, L.F64, D.Pencil(), (1, 2, 3))
+    q = [x for x in p.passes() if x["axis"] == ax][0]
+    assert q["L"] == shape[ax], q
+    if ax > 0:
+        assert q["swz"] == -8, q
+    A = D.DeviceArray.from_host(a)
+    F = p.execute(A, D.DeviceArray(shape, np.complex128))
+    del A
+    y = F.to_host()
+    e = O.rel_l2(y, O.fftn(a))
+    del y
+    pi = D.Plan(shape, L.C2C, +1, L.F64, D.Pencil(), (1, 2, 3))
+    B = pi.execute(F, F.similar())
+    del F
+    e2 = O.rel_l2(B.to_host(), a)
+    print("rel-L2 fwd %.3e, fwd+inv %.3e" % (e, e2))
+    assert e <= 1e-12 and e2 <= 1e-12, (shape, e, e2)
 
 
 def test_longest_lines(gpu):
