@@ -750,12 +750,49 @@ __device__ __forceinline__ void r2c_post_split(const TileCtx<T, OP, CONTIG> &c, 
     }
 }
 
+// Stage-0 loads of a C2C tile: the positions st_load_global reads, with the
+// C2C load map of TileCtx::load0 reordered so that every load is issued
+// before any loaded value is used.  Lines past the array's end read line 0
+// of the pass (in bounds; those lines compute on it and are never stored)
+// instead of taking zeros, and the inverse transform's conjugate follows the
+// loads.  With the map's validity select and conjugate after each load, the
+// compiler waited for every pair of loads before issuing the next.
+template <int R, int EMAX, int LC, int TPLC, class C>
+__device__ __forceinline__ void c2c_load_global(const C &c, cx<typename C::value_type> *v) {
+    typedef typename C::value_type T;
+    constexpr int NBM = EMAX / R;
+    const int NB = LC / R;
+    const cx<T> *inc = reinterpret_cast<const cx<T> *>(c.in) + (c.valid ? c.ib : 0);
+#pragma unroll
+    for (int b = 0; b < NBM; ++b) {
+        const int j = c.t + b * TPLC;
+        if (j < NB) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b * R + r] = gload(inc + (int64_t)(j + r * NB) * c.a.in_sm, c.a.nt);
+        }
+    }
+    if (c.a.inverse) {
+#pragma unroll
+        for (int b = 0; b < NBM; ++b) {
+            if (c.t + b * TPLC < NB) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[b * R + r] = conj(v[b * R + r]);
+            }
+        }
+    }
+}
+
 // All stages of a fixed schedule: global -> registers, LDS exchanges, registers -> global.
 template <class RL, int EMAX, int TPLC, bool SPLIT, class T, int OP, bool CONTIG>
 __device__ __forceinline__ void run_fixed_stages(const TileCtx<T, OP, CONTIG> &c) {
     constexpr int L = RL::L, R0 = RL::R[0], RN = RL::R[RL::S - 1];
     cx<T> v[EMAX];
-    if constexpr (OP == OP_DCT2 && CONTIG && dct_pairs_ok<R0, EMAX, L, TPLC>())
+    if constexpr (OP == OP_C2C && sizeof(T) == 4) c2c_load_global<R0, EMAX, L, TPLC>(c, v);
+    else if constexpr (OP == OP_C2C) {
+        if (c.a.paired_loads) st_load_global<R0, EMAX, L, TPLC>(c, v);
+        else c2c_load_global<R0, EMAX, L, TPLC>(c, v);
+    }
+    else if constexpr (OP == OP_DCT2 && CONTIG && dct_pairs_ok<R0, EMAX, L, TPLC>())
         st_load_dct2_pairs<R0, EMAX, L, TPLC>(c, v);
     else if constexpr (OP == OP_DCT2H && CONTIG && dct_pairs_ok<R0, EMAX, L, TPLC>())
         st_load_dct2h_quads<R0, EMAX, L, TPLC>(c, v);

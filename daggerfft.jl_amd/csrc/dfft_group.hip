@@ -994,6 +994,7 @@ int dfft_batch_create(dfft_batch **batch, int count, const dfft_batch_desc *desc
                 a.scale = (inv && norm) ? 1.0 / L : 1.0;
                 set_geometry(a, ax, gsz, gsz);
                 a.nt = 1;
+                a.paired_loads = paired_loads_for(a.in_sm, prec ? 16 : 8);
                 pp.alg_bytes = 2.0 * (d.kind == DFFT_R2C ? NC : N) * es_c;
             }
             b->alg_bytes += pp.alg_bytes;

@@ -699,6 +699,7 @@ int build_plan(dfft_plan *p) {
         a.sp.L = L;
         a.nt = 1;  // every element is read once and written once per pass
         set_geometry(a, axis, nin, nout);
+        a.paired_loads = paired_loads_for(a.in_sm, es_c);
         p->passes.push_back(pp);
     };
 
@@ -1231,10 +1232,10 @@ int dfft_plan_describe(const dfft_plan *p, int pass, char *buf, size_t buflen) {
     snprintf(buf, buflen,
              "{\"pass\":%d,\"axis\":%d,\"op\":\"%s\",\"L\":%d,\"nlog\":%d,\"radices\":\"%s\",\"TPL\":%d,\"TC\":%d,"
              "\"contig\":%d,\"variant\":%d,\"threads\":%d,\"blocks\":%lld,\"lds\":%zu,\"bytes\":%.0f,\"prec\":\"%s\","
-             "\"swz\":%d,\"bluestein\":%d,\"rader\":%d,\"kernel\":\"%s\"}",
+             "\"swz\":%d,\"bluestein\":%d,\"rader\":%d,\"paired_loads\":%d,\"kernel\":\"%s\"}",
              pass, pp.axis, op_name(a.op), a.sp.L, a.nlog, rad.c_str(), a.TPL, a.TC, a.contig, a.variant, pp.threads,
              (long long)pp.nblocks, pp.lds, pp.alg_bytes, pp.prec ? "f64" : "f32", a.swz, a.blue == 1 ? a.bsp.L : 0,
-             a.blue == 2 ? a.bsp.L : 0, kname.c_str());
+             a.blue == 2 ? a.bsp.L : 0, a.paired_loads, kname.c_str());
     return DFFT_OK;
 }
 

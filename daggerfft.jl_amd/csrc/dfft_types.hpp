@@ -59,6 +59,10 @@ struct AxisPass {
     int pad_shift;            // contig: slot index p -> p + (p >> pad_shift)
     int nlog;                 // logical line length (real length for R2C/C2R/R2R)
     int nt;                   // 1: streaming (nontemporal) global loads/stores
+    int paired_loads;         // ComplexF64 C2C on the compile-time engine: 1 issues the first
+                              // stage's loads in the load map's order (a couple in flight per
+                              // wave at a time), 0 issues them all before any is used
+                              // (paired_loads_for)
     int variant;              // 0: runtime schedule (any length); 1: compile-time schedule of this
                               // length, 8 values/thread; 2: compile-time schedule, 16 values/thread,
                               // real and imaginary parts exchanged in two rounds (half the LDS);
@@ -86,5 +90,19 @@ struct AxisPass {
                               // bsp's radices reversed (its first stage takes the first FFT's
                               // last-stage outputs straight from registers)
 };
+
+// AxisPass::paired_loads for a pass whose lines have stride `in_sm`
+// elements of `elem_bytes`: the first stage's loads are all issued at once
+// (faster on every measured shape: 3-9% on 384x240x210, the 64-transform
+// batch and the served mix) except along ComplexF64 lines whose stride is a
+// power of two of 8 KiB or more, where 512^3's strided passes measured 2-3%
+// slower that way.  ComplexF32 kernels hold only the all-at-once order:
+// with both orders the 512-point strided kernel took 65 VGPRs, one past the
+// 64 that let two 1024-thread workgroups share a CU, and ran 24% slower,
+// while all at once alone matched the paired order on 512^3.
+inline int paired_loads_for(int64_t in_sm, int elem_bytes) {
+    const int64_t b = in_sm * (int64_t)elem_bytes;
+    return (elem_bytes == 16 && b >= 8192 && (b & (b - 1)) == 0) ? 1 : 0;
+}
 
 }  // namespace dfft

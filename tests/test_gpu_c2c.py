@@ -185,3 +185,33 @@ def test_slab_plane_pipeline_many_chunks(gpu, monkeypatch, shape, planes, prec):
     spec = np.asfortranarray(ref.astype(cdt))
     Br = pc.execute(_dev(spec), D.DeviceArray(shape, rdt))
     chk(Br.to_host(), O.irfftn(spec.astype(np.complex128), shape[0]))
+
+
+def test_first_stage_load_order(gpu):
+    """The first stage's loads of a catalog C2C pass are all issued at once,
+    except along ComplexF64 lines whose stride is a power of two of 8 KiB or
+    more (AxisPass::paired_loads, shown by dfft_plan_describe).  A plan whose
+    passes take both orders matches the oracle forward and back; ComplexF32
+    takes the all-at-once order on every pass."""
+    import daggerfft as D
+    from daggerfft import _lib as L
+    expect = {
+        ((512, 512, 512), L.F64): {0: 0, 1: 1, 2: 1},      # strides 16 B, 8 KiB, 4 MiB
+        ((512, 512, 512), L.F32): {0: 0, 1: 0, 2: 0},      # ComplexF32: always all at once
+        ((384, 240, 210), L.F64): {0: 0, 1: 0, 2: 0},      # 6 KiB, 1.4 MiB: not powers of two
+    }
+    for (shape, prec), want in expect.items():
+        p = D.Plan(shape, L.C2C, -1, prec, D.Pencil(), (1, 2, 3))
+        got = {q["axis"]: q["paired_loads"] for q in p.passes()}
+        assert got == want, (shape, prec, got)
+    for prec, dt, tol in ((L.F64, np.complex128, TOL64), (L.F32, np.complex64, TOL32)):
+        shape = (32, 32, 48)  # axis 3: 1024 elements apart (16 KiB in ComplexF64), axis 2: 32
+        p = D.Plan(shape, L.C2C, -1, prec, D.Pencil(), (1, 2, 3))
+        orders = {q["axis"]: q["paired_loads"] for q in p.passes()}
+        assert orders == {0: 0, 1: 0, 2: 1 if prec == L.F64 else 0}, orders
+        a = O.fill(61, shape, dt)
+        F = p.execute(D.DeviceArray.from_host(a), D.DeviceArray(shape, dt))
+        e = O.rel_l2(F.to_host().astype(np.complex128), O.fftn(a.astype(np.complex128)))
+        B = D.Plan(shape, L.C2C, +1, prec, D.Pencil(), (1, 2, 3)).execute(F, F.similar())
+        e2 = O.rel_l2(B.to_host().astype(np.complex128), a.astype(np.complex128))
+        assert e <= tol and e2 <= tol, (prec, e, e2)
