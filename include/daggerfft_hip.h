@@ -182,8 +182,11 @@ int dfft_execute_pass(const dfft_plan *plan, int pass, const void *d_in, void *d
  * dfft_plan_describe writes one JSON object per pass: axis, op, L (line
  * length), radices, TPL/TC (threads per line, lines or columns per tile),
  * variant (0 runtime schedule, >= 1 compile-time), launch shape, LDS bytes,
- * algorithmic bytes, bluestein (the convolution length of a Bluestein pass,
- * 0 otherwise) and the kernel's name as profilers print it. */
+ * algorithmic bytes, swz (tile order), bluestein / rader (the convolution
+ * length of a Bluestein or Rader pass, 0 otherwise), paired_loads (1: the
+ * first stage's loads of a ComplexF64 C2C pass along a power-of-two stride of
+ * 8 KiB or more go out a pair at a time, 0: all at once) and the kernel's
+ * name as profilers print it. */
 int dfft_plan_num_passes(const dfft_plan *plan, int *npasses);
 int dfft_plan_describe(const dfft_plan *plan, int pass, char *buf, size_t buflen);
 
