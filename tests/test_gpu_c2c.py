@@ -215,3 +215,30 @@ def test_first_stage_load_order(gpu):
         B = D.Plan(shape, L.C2C, +1, prec, D.Pencil(), (1, 2, 3)).execute(F, F.similar())
         e2 = O.rel_l2(B.to_host().astype(np.complex128), a.astype(np.complex128))
         assert e <= tol and e2 <= tol, (prec, e, e2)
+
+
+@pytest.mark.parametrize("prec", [1, 0], ids=["f64", "f32"])
+def test_inplace_partial_tiles(gpu, prec):
+    """In place, with tiles that run past the array's last line: the first
+    stage of a C2C pass, and the copy-in of a Rader pass, read line 0 of the
+    pass (or element 0) for the lines past the end rather than zeros -- values
+    that are never stored, even while line 0 is being overwritten by its own
+    tile.  Forward then inverse in place, against the oracle, on shapes whose
+    passes end in partial tiles (catalog lengths and a Rader length, 257)."""
+    import daggerfft as D
+    from daggerfft import _lib as L
+    dt = np.complex128 if prec else np.complex64
+    tol = TOL64 if prec else TOL32
+    for shape in ((96, 33, 7), (7, 96, 33), (257, 5, 3), (5, 257, 3)):
+        print("in place, partial tiles", shape, "f64" if prec else "f32", flush=True)
+        p = D.Plan(shape, L.C2C, -1, prec, D.Pencil(), (1, 2, 3))
+        n = int(np.prod(shape))
+        partial = [q["axis"] for q in p.passes() if q["blocks"] * q["TC"] * q["L"] != n]
+        assert partial, (shape, p.passes())
+        a = O.fill(71, shape, dt)
+        A = D.DeviceArray.from_host(a)
+        p.execute(A, A)
+        e = O.rel_l2(A.to_host().astype(np.complex128), O.fftn(a.astype(np.complex128)))
+        D.Plan(shape, L.C2C, +1, prec, D.Pencil(), (1, 2, 3)).execute(A, A)
+        e2 = O.rel_l2(A.to_host().astype(np.complex128), a.astype(np.complex128))
+        assert e <= tol and e2 <= tol, (shape, e, e2)
