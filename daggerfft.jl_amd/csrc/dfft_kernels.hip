@@ -2,7 +2,8 @@
 //
 //   k_axis      one Stockham axis pass per launch (C2C / R2C / C2R / every r2r kind);
 //               catalog kernels in dfft_fixed_*.hip, Bluestein kernels in dfft_blue.hip
-//   k_fill      SplitMix64 counter-based uniform[0,1) input generator
+//   k_fill      SplitMix64 counter-based uniform[0,1) input generator (k_fill16:
+//               the same values, 16 bytes per lane per store)
 //   k_box       box gather / scatter / copy (pack_kernel!, unpack_kernel!,
 //               local_copy_kernel!; src/fftgpu.jl:452-495), coalesced along axis 0
 //   k_permute   LDS-tiled 3D axis permutation (the pencil transpose)
@@ -152,8 +153,41 @@ __global__ __launch_bounds__(256) void k_fill(T *out, int64_t nscalar, uint64_t 
         out[i] = (T)splitmix_u(seed, (uint64_t)i);
 }
 
+// The same values, 16 bytes per lane per store: thread i writes scalars
+// V i .. V i + V - 1 (V = 4 floats or 2 doubles) of a 16-byte aligned `out`;
+// the scalars past the last whole vector are written one per lane.  With
+// one 4-byte store per lane, the ComplexF32 inputs of the served stream
+// filled at well under the write bandwidth.
+template <class T>
+struct alignas(16) FillVec {
+    T v[16 / sizeof(T)];
+};
+template <class T>
+__global__ __launch_bounds__(256) void k_fill16(T *out, int64_t nscalar, uint64_t seed) {
+    constexpr int V = 16 / sizeof(T);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x, nvec = nscalar / V;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    FillVec<T> *o = reinterpret_cast<FillVec<T> *>(out);
+    for (int64_t i = t; i < nvec; i += stride) {
+        FillVec<T> f;
+#pragma unroll
+        for (int k = 0; k < V; ++k) f.v[k] = (T)splitmix_u(seed, (uint64_t)(i * V + k));
+        o[i] = f;
+    }
+    for (int64_t i = nvec * V + t; i < nscalar; i += stride) out[i] = (T)splitmix_u(seed, (uint64_t)i);
+}
+
 hipError_t launch_fill(void *out, int64_t nscalar, int prec, uint64_t seed, hipStream_t stream) {
     if (nscalar <= 0) return hipSuccess;
+    if (((uintptr_t)out & 15) == 0) {
+        const int V = prec ? 2 : 4;
+        int64_t blocks = (nscalar / V + 255) / 256;
+        if (blocks < 1) blocks = 1;
+        if (blocks > 65536) blocks = 65536;
+        if (prec) hipLaunchKernelGGL(k_fill16<double>, dim3((unsigned)blocks), dim3(256), 0, stream, (double *)out, nscalar, seed);
+        else hipLaunchKernelGGL(k_fill16<float>, dim3((unsigned)blocks), dim3(256), 0, stream, (float *)out, nscalar, seed);
+        return hipGetLastError();
+    }
     int64_t blocks = (nscalar + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     if (prec) hipLaunchKernelGGL(k_fill<double>, dim3((unsigned)blocks), dim3(256), 0, stream, (double *)out, nscalar, seed);

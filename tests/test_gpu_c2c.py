@@ -21,9 +21,28 @@ def _dev(a):
 def test_fill_bit_exact(gpu):
     import daggerfft as D
     for dt in (np.complex128, np.complex64, np.float64, np.float32):
-        A = D.rand((17, 5, 3), dt, seed=7)
-        ref = O.fill(7, (17, 5, 3), dt)
-        assert np.array_equal(A.to_host(), ref), dt
+        for shape in ((17, 5, 3), (1001, 999, 1), (3,)):
+            A = D.rand(shape, dt, seed=7)
+            ref = O.fill(7, shape, dt)
+            assert np.array_equal(A.to_host(), ref), (dt, shape)
+
+
+def test_fill_unaligned_pointer(gpu):
+    """dfft_fill_uniform at an address that is not 16-byte aligned (one
+    scalar into a buffer) takes the one-scalar-per-lane kernel; the values
+    are the same as at an aligned address."""
+    import ctypes
+    import torch
+    from daggerfft import _lib as L
+    for dt, prec, cplx in ((np.complex128, 1, 1), (np.complex64, 0, 1), (np.float64, 1, 0), (np.float32, 0, 0)):
+        n = 100003
+        es = np.dtype(dt).itemsize // (2 if cplx else 1)
+        buf = torch.zeros((n + 4) * np.dtype(dt).itemsize, dtype=torch.uint8, device="cuda")
+        L.check(L.lib().dfft_fill_uniform(ctypes.c_void_p(buf.data_ptr() + es), n, cplx, prec, ctypes.c_uint64(9),
+                                          None), "dfft_fill_uniform")
+        torch.cuda.synchronize()
+        got = buf.cpu().numpy()[es:es + n * np.dtype(dt).itemsize].view(dt)
+        assert np.array_equal(got, O.fill(9, (n,), dt)), dt
 
 
 @pytest.mark.parametrize("shape", [(8, 8, 8), (16, 12, 10), (7, 5, 3), (64, 48, 30), (128, 128, 128),
